@@ -1,0 +1,674 @@
+// HIP kernels for gfx950 (CDNA4): batched 1-D Stockham FFT passes over
+// distributed arrays, the exchange remap, and the reference-exact pack /
+// unpack / all-to-all kernels.
+//
+// FFT conventions (the ones kokkos-fft exposes to the reference plans,
+// KokkosFFT_Distributed_SlabPlan.hpp:978-1081): forward X[k] = sum_j x[j]
+// exp(-2 pi i jk/n), backward with +i, both unscaled; scaling is applied by
+// the caller-supplied factor at the final store.  Inverse transforms run the
+// forward kernel between two conjugations (IDFT(x) = conj(DFT(conj(x)))).
+// R2C/C2R lines are carried as full complex lines of length n (real input
+// with zero imaginary part; Hermitian extension of the n/2+1 input values).
+#include <hip/hip_runtime.h>
+
+#include "fft_kernels.hpp"
+
+namespace dfft {
+
+// ---------------------------------------------------------------------------
+// Radix plans of the compile-time kernels.
+// ---------------------------------------------------------------------------
+struct FPlan {
+  int R = 1;    // complex values each thread keeps in registers
+  int nst = 0;  // number of stages
+  int rad[12] = {};
+  bool ok = true;
+};
+
+constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
+
+constexpr FPlan make_plan(int n) {
+  FPlan p;
+  if (n <= 1) return p;
+  int m = n;
+  while (m % 8 == 0) { p.rad[p.nst++] = 8; m /= 8; }
+  if (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
+  if (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
+  while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; }
+  p.ok = (m == 1);
+  int R = 1;
+  for (int s = 0; s < p.nst; ++s) R = R / cgcd(R, p.rad[s]) * p.rad[s];
+  p.R = R;
+  return p;
+}
+
+constexpr int span_before(int n, int s) {
+  FPlan p = make_plan(n);
+  int L = 1;
+  for (int i = 0; i < s; ++i) L *= p.rad[i];
+  return L;
+}
+
+// ---------------------------------------------------------------------------
+// Forward DFT butterflies on registers (exponent -2 pi i jk/r).
+// ---------------------------------------------------------------------------
+template <typename C> __device__ __forceinline__ void bfly2(C* a) {
+  C t = a[0];
+  a[0] = cadd(t, a[1]);
+  a[1] = csub(t, a[1]);
+}
+
+template <typename C> __device__ __forceinline__ void bfly4(C* a) {
+  C t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+  C t2 = cadd(a[1], a[3]), t3 = cmul_mi(csub(a[1], a[3]));
+  a[0] = cadd(t0, t2);
+  a[2] = csub(t0, t2);
+  a[1] = cadd(t1, t3);
+  a[3] = csub(t1, t3);
+}
+
+template <typename C> __device__ __forceinline__ void bfly8(C* a) {
+  using R = decltype(a[0].x);
+  const R h = static_cast<R>(0.70710678118654752440);  // 1/sqrt(2)
+  C e[4] = {a[0], a[2], a[4], a[6]};
+  C o[4] = {a[1], a[3], a[5], a[7]};
+  bfly4(e);
+  bfly4(o);
+  // o[k] *= exp(-2 pi i k / 8)
+  C o1 = C{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
+  C o2 = cmul_mi(o[2]);
+  C o3 = C{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
+  a[0] = cadd(e[0], o[0]);
+  a[4] = csub(e[0], o[0]);
+  a[1] = cadd(e[1], o1);
+  a[5] = csub(e[1], o1);
+  a[2] = cadd(e[2], o2);
+  a[6] = csub(e[2], o2);
+  a[3] = cadd(e[3], o3);
+  a[7] = csub(e[3], o3);
+}
+
+template <typename C> __device__ __forceinline__ void bfly3(C* a) {
+  using R = decltype(a[0].x);
+  const R s3 = static_cast<R>(0.86602540378443864676);  // sin(2 pi / 3)
+  C s = cadd(a[1], a[2]);
+  C d = csub(a[1], a[2]);
+  C m = C{a[0].x - static_cast<R>(0.5) * s.x, a[0].y - static_cast<R>(0.5) * s.y};
+  C t = cmul_mi(C{s3 * d.x, s3 * d.y});
+  a[0] = cadd(a[0], s);
+  a[1] = cadd(m, t);
+  a[2] = csub(m, t);
+}
+
+template <typename C> __device__ __forceinline__ void bfly5(C* a) {
+  using R = decltype(a[0].x);
+  const R c1 = static_cast<R>(0.30901699437494742410);   // cos(2 pi/5)
+  const R c2 = static_cast<R>(-0.80901699437494742410);  // cos(4 pi/5)
+  const R s1 = static_cast<R>(0.95105651629515357212);   // sin(2 pi/5)
+  const R s2 = static_cast<R>(0.58778525229247312917);   // sin(4 pi/5)
+  C b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
+  C d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+  C A1 = C{a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y};
+  C A2 = C{a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y};
+  C X1 = cmul_mi(C{s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y});
+  C X2 = cmul_mi(C{s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y});
+  a[0] = cadd(a[0], cadd(b1, b2));
+  a[1] = cadd(A1, X1);
+  a[4] = csub(A1, X1);
+  a[2] = cadd(A2, X2);
+  a[3] = csub(A2, X2);
+}
+
+template <int r, typename C> __device__ __forceinline__ void bfly(C* a) {
+  if constexpr (r == 2) bfly2(a);
+  else if constexpr (r == 3) bfly3(a);
+  else if constexpr (r == 4) bfly4(a);
+  else if constexpr (r == 5) bfly5(a);
+  else if constexpr (r == 8) bfly8(a);
+}
+
+// ---------------------------------------------------------------------------
+// Element access of a line.
+// ---------------------------------------------------------------------------
+template <typename Real> struct LineIO {
+  using C = cplx_t<Real>;
+  const char* ib;
+  char* ob;
+  int64_t is, os;
+  int32_t kind, n, inverse;
+  Real scale;
+
+  __device__ __forceinline__ C load(int32_t pos) const {
+    C v;
+    if (kind == kC2C) {
+      v = reinterpret_cast<const C*>(ib)[pos * is];
+    } else if (kind == kR2C) {
+      v = C{reinterpret_cast<const Real*>(ib)[pos * is], Real(0)};
+    } else {
+      const int32_t nh = n / 2 + 1;
+      if (pos < nh) {
+        v = reinterpret_cast<const C*>(ib)[pos * is];
+      } else {
+        v = cconj(reinterpret_cast<const C*>(ib)[(n - pos) * is]);
+      }
+      if (pos == 0 || 2 * pos == n) v.y = Real(0);
+    }
+    return inverse ? cconj(v) : v;
+  }
+  __device__ __forceinline__ void store(int32_t pos, C v) const {
+    if (inverse) v = cconj(v);
+    v.x *= scale;
+    v.y *= scale;
+    if (kind == kC2C) {
+      reinterpret_cast<C*>(ob)[pos * os] = v;
+    } else if (kind == kR2C) {
+      if (2 * pos <= n) reinterpret_cast<C*>(ob)[pos * os] = v;
+    } else {
+      reinterpret_cast<Real*>(ob)[pos * os] = v.x;
+    }
+  }
+};
+
+template <typename Real>
+__device__ __forceinline__ void decode_line(const LinePass& p, int32_t line, int32_t* idx) {
+#pragma unroll
+  for (int a = 0; a < kMaxDim; ++a) idx[a] = 0;
+  uint32_t rem = static_cast<uint32_t>(line);
+#pragma unroll
+  for (int j = kMaxDim - 2; j >= 0; --j) {
+    if (j < p.nbatch) {
+      uint32_t q = p.batch_div[j].div(rem);
+      idx[p.batch_axis[j]] = static_cast<int32_t>(rem - q * static_cast<uint32_t>(p.batch_ext[j]));
+      rem = q;
+    }
+  }
+}
+
+template <typename Real>
+__device__ __forceinline__ LineIO<Real> make_line_io(const LinePass& p, int32_t line) {
+  using C = cplx_t<Real>;
+  int32_t idx[kMaxDim];
+  decode_line<Real>(p, line, idx);
+  LineIO<Real> io;
+  const int64_t ies = p.kind == kR2C ? sizeof(Real) : sizeof(C);
+  const int64_t oes = p.kind == kC2R ? sizeof(Real) : sizeof(C);
+  io.ib = element_ptr(p.in, idx, ies, p.axis, io.is);
+  io.ob = element_ptr(p.out, idx, oes, p.axis, io.os);
+  io.kind = p.kind;
+  io.n = p.n;
+  io.inverse = p.inverse;
+  io.scale = static_cast<Real>(p.scale);
+  return io;
+}
+
+// ---------------------------------------------------------------------------
+// Compile-time-length Stockham kernel: each line is processed by T = N/R
+// threads holding R values in registers; radix stages exchange through LDS.
+// ---------------------------------------------------------------------------
+template <typename Real> struct LdsPad;
+template <> struct LdsPad<double> { static constexpr int shift = 3; };
+template <> struct LdsPad<float> { static constexpr int shift = 4; };
+
+template <typename Real, int N> struct FastCfg {
+  static constexpr FPlan P = make_plan(N);
+  static constexpr int R = P.R;
+  static constexpr int T = N / R;
+  static constexpr int NST = P.nst;
+  static constexpr int LDSN = N + (N >> LdsPad<Real>::shift) + 1;
+  __device__ static __forceinline__ int lidx(int pos) { return pos + (pos >> LdsPad<Real>::shift); }
+};
+
+template <typename Real, int N, int S>
+__device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const cplx_t<Real>* tw) {
+  using Cfg = FastCfg<Real, N>;
+  constexpr int r = Cfg::P.rad[S];
+  constexpr int L = span_before(N, S);
+  constexpr int R = Cfg::R;
+  constexpr int NB = R / r;
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    if constexpr (L > 1) {
+      const int beta = t * NB + q;
+      const int k = beta % L;
+      constexpr int ts = N / (L * r);
+#pragma unroll
+      for (int j = 1; j < r; ++j) v[q * r + j] = cmul(v[q * r + j], tw[j * k * ts]);
+    }
+    bfly<r>(v + q * r);
+  }
+}
+
+// write the outputs of stage S, read the inputs of stage S+1
+template <typename Real, int N, int S>
+__device__ __forceinline__ void stage_exchange(cplx_t<Real>* v, int t, cplx_t<Real>* line_lds) {
+  using Cfg = FastCfg<Real, N>;
+  constexpr int r = Cfg::P.rad[S];
+  constexpr int L = span_before(N, S);
+  constexpr int R = Cfg::R;
+  constexpr int NB = R / r;
+  constexpr int r2 = Cfg::P.rad[S + 1];
+  constexpr int NB2 = R / r2;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int beta = t * NB + q;
+    const int k = beta % L;
+    const int base = (beta / L) * L * r + k;
+#pragma unroll
+    for (int j = 0; j < r; ++j) line_lds[Cfg::lidx(base + j * L)] = v[q * r + j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NB2; ++q) {
+    const int beta = t * NB2 + q;
+#pragma unroll
+    for (int j = 0; j < r2; ++j) v[q * r2 + j] = line_lds[Cfg::lidx(beta + j * (N / r2))];
+  }
+}
+
+template <typename Real, int N, int S> struct StageLoop {
+  __device__ static __forceinline__ void run(cplx_t<Real>* v, int t, const cplx_t<Real>* tw,
+                                             cplx_t<Real>* line_lds) {
+    stage_compute<Real, N, S>(v, t, tw);
+    if constexpr (S + 1 < FastCfg<Real, N>::NST) {
+      stage_exchange<Real, N, S>(v, t, line_lds);
+      StageLoop<Real, N, S + 1>::run(v, t, tw, line_lds);
+    }
+  }
+};
+
+template <typename Real, int N>
+__global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
+  using C = cplx_t<Real>;
+  using Cfg = FastCfg<Real, N>;
+  constexpr int R = Cfg::R, T = Cfg::T, NST = Cfg::NST;
+  extern __shared__ unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+  const int B = p.lines_per_block;
+  const int tid = threadIdx.x;
+  int b, t;
+  if (p.lines_fast) {
+    b = tid % B;
+    t = tid / B;
+  } else {
+    t = tid % T;
+    b = tid / T;
+  }
+  const int32_t line = blockIdx.x * B + b;
+  const bool valid = line < p.nlines;
+  const C* tw = reinterpret_cast<const C*>(p.tw);
+  C* line_lds = lds + b * Cfg::LDSN;
+
+  LineIO<Real> io;
+  if (valid) io = make_line_io<Real>(p, line);
+
+  C v[R];
+  {
+    constexpr int r0 = Cfg::P.rad[0];
+    constexpr int NB0 = R / r0;
+#pragma unroll
+    for (int q = 0; q < NB0; ++q) {
+      const int beta = t * NB0 + q;
+#pragma unroll
+      for (int j = 0; j < r0; ++j) v[q * r0 + j] = valid ? io.load(beta + j * (N / r0)) : C{Real(0), Real(0)};
+    }
+  }
+  StageLoop<Real, N, 0>::run(v, t, tw, line_lds);
+
+  constexpr int rl = Cfg::P.rad[NST - 1];
+  if constexpr (rl != R) {
+    // outputs sit at positions beta + j*(N/rl); regroup to t + j*T through LDS
+    constexpr int NBl = R / rl;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NBl; ++q) {
+      const int beta = t * NBl + q;
+#pragma unroll
+      for (int j = 0; j < rl; ++j) line_lds[Cfg::lidx(beta + j * (N / rl))] = v[q * rl + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = line_lds[Cfg::lidx(t + j * T)];
+  }
+  if (valid) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) io.store(t + j * T, v[j]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Generic runtime-length kernel (any n): lines staged in LDS, every output of
+// a radix-r stage is a length-r sum with combined twiddles.  Handles lines
+// that cross rank boundaries (per-element addressing).
+// ---------------------------------------------------------------------------
+struct GenPlan {
+  int32_t nst;
+  int32_t rad[32];
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const GenPlan g) {
+  using C = cplx_t<Real>;
+  extern __shared__ unsigned char smem_raw[];
+  const int32_t n = p.n;
+  const int B = p.lines_per_block;
+  C* bufA = reinterpret_cast<C*>(smem_raw);
+  C* bufB = bufA + B * n;
+  const C* tw = reinterpret_cast<const C*>(p.tw);
+  const int32_t total = B * n;
+  const int64_t ies = p.kind == kR2C ? sizeof(Real) : sizeof(C);
+  const int64_t oes = p.kind == kC2R ? sizeof(Real) : sizeof(C);
+
+  for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int32_t b = e / n, pos = e - b * n;
+    const int32_t line = blockIdx.x * B + b;
+    C v{Real(0), Real(0)};
+    if (line < p.nlines) {
+      int32_t idx[kMaxDim];
+      decode_line<Real>(p, line, idx);
+      LineIO<Real> io;
+      io.kind = p.kind;
+      io.n = n;
+      io.inverse = p.inverse;
+      io.scale = static_cast<Real>(p.scale);
+      if (p.kind == kC2R && pos >= n / 2 + 1) {
+        idx[p.axis] = n - pos;
+      } else {
+        idx[p.axis] = pos;
+      }
+      int64_t st;
+      io.ib = element_ptr(p.in, idx, ies, p.axis, st);
+      io.is = 0;
+      // LineIO::load indexes pos*is from ib; with is = 0 the element pointer is used as is,
+      // except the Hermitian extension which needs the conjugate of X[n-pos].
+      if (p.kind == kC2R) {
+        v = *reinterpret_cast<const C*>(io.ib);
+        if (pos >= n / 2 + 1) v = cconj(v);
+        if (pos == 0 || 2 * pos == n) v.y = Real(0);
+        if (p.inverse) v = cconj(v);
+      } else {
+        v = io.load(0);
+      }
+    }
+    bufA[e] = v;
+  }
+  __syncthreads();
+  int32_t L = 1;
+  for (int s = 0; s < g.nst; ++s) {
+    const int32_t r = g.rad[s];
+    const int32_t Lr = L * r;
+    const int32_t ts = n / Lr;
+    const int32_t nr = n / r;
+    for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
+      const int32_t b = e / n, o = e - b * n;
+      const int32_t grp = o / Lr, rm = o - grp * Lr;
+      const int32_t q = rm / L, k = rm - q * L;
+      const int32_t beta = grp * L + k;
+      const C* src = bufA + b * n;
+      C acc{Real(0), Real(0)};
+      const int32_t step = k + q * L;  // exponent step in units of W_{Lr}
+      int32_t ex = 0;
+      for (int32_t j = 0; j < r; ++j) {
+        C x = src[beta + j * nr];
+        acc = cadd(acc, cmul(x, tw[ex * ts]));
+        ex += step;
+        if (ex >= Lr) ex -= Lr * (ex / Lr);
+      }
+      bufB[e] = acc;
+    }
+    __syncthreads();
+    C* tmp = bufA;
+    bufA = bufB;
+    bufB = tmp;
+    L = Lr;
+  }
+  for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int32_t b = e / n, pos = e - b * n;
+    const int32_t line = blockIdx.x * B + b;
+    if (line >= p.nlines) continue;
+    if (p.kind == kR2C && 2 * pos > n) continue;
+    int32_t idx[kMaxDim];
+    decode_line<Real>(p, line, idx);
+    idx[p.axis] = pos;
+    int64_t st;
+    LineIO<Real> io;
+    io.kind = p.kind;
+    io.n = n;
+    io.inverse = p.inverse;
+    io.scale = static_cast<Real>(p.scale);
+    io.ob = element_ptr(p.out, idx, oes, p.axis, st);
+    io.os = 0;
+    io.store(0, bufA[e]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename Real, int N>
+hipError_t launch_fast(LinePass p, hipStream_t s) {
+  using Cfg = FastCfg<Real, N>;
+  using C = cplx_t<Real>;
+  constexpr int T = Cfg::T;
+  int B;
+  if (p.lines_fast) {
+    const int coalesce = 128 / static_cast<int>(sizeof(C));  // lines per 128-byte row segment
+    B = coalesce;
+    while (B * T > 1024 && B > 1) B /= 2;
+  } else {
+    B = T >= 256 ? 1 : 256 / T;
+  }
+  while (B > 1 && static_cast<size_t>(B) * Cfg::LDSN * sizeof(C) > 160 * 1024) B /= 2;
+  p.lines_per_block = B;
+  const size_t smem = static_cast<size_t>(B) * Cfg::LDSN * sizeof(C);
+  if (smem > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_line_fast<Real, N>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
+    if (e != hipSuccess) return e;
+  }
+  const unsigned grid = static_cast<unsigned>((p.nlines + B - 1) / B);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((fft_line_fast<Real, N>), dim3(grid), dim3(T * B), smem, s, p);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_generic(LinePass p, hipStream_t s) {
+  using C = cplx_t<Real>;
+  GenPlan g{};
+  int m = p.n;
+  g.nst = 0;
+  while (m % 4 == 0 && m > 4) { g.rad[g.nst++] = 4; m /= 4; }
+  for (int f = 2; m > 1 && g.nst < 32;) {
+    if (m % f == 0) {
+      g.rad[g.nst++] = f;
+      m /= f;
+    } else {
+      ++f;
+      if (f * f > m) f = m;
+    }
+  }
+  if (m != 1) return hipErrorInvalidValue;
+  int B = p.n >= 1024 ? 1 : 1024 / p.n;
+  while (B > 1 && static_cast<size_t>(2) * B * p.n * sizeof(C) > 64 * 1024) B /= 2;
+  const size_t smem = static_cast<size_t>(2) * B * p.n * sizeof(C);
+  if (smem > 160 * 1024) return hipErrorInvalidValue;
+  if (smem > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_line_generic<Real>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
+    if (e != hipSuccess) return e;
+  }
+  p.lines_per_block = B;
+  const unsigned grid = static_cast<unsigned>((p.nlines + B - 1) / B);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((fft_line_generic<Real>), dim3(grid), dim3(256), smem, s, p, g);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t dispatch(const LinePass& p, hipStream_t s) {
+  switch (p.n) {
+    case 2: return launch_fast<Real, 2>(p, s);
+    case 4: return launch_fast<Real, 4>(p, s);
+    case 8: return launch_fast<Real, 8>(p, s);
+    case 16: return launch_fast<Real, 16>(p, s);
+    case 32: return launch_fast<Real, 32>(p, s);
+    case 64: return launch_fast<Real, 64>(p, s);
+    case 96: return launch_fast<Real, 96>(p, s);
+    case 128: return launch_fast<Real, 128>(p, s);
+    case 160: return launch_fast<Real, 160>(p, s);
+    case 256: return launch_fast<Real, 256>(p, s);
+    case 512: return launch_fast<Real, 512>(p, s);
+    case 1024: return launch_fast<Real, 1024>(p, s);
+    case 2048: return launch_fast<Real, 2048>(p, s);
+    default: return launch_generic<Real>(p, s);
+  }
+}
+
+}  // namespace
+
+bool has_fast_line_kernel(int n) {
+  switch (n) {
+    case 2: case 4: case 8: case 16: case 32: case 64: case 96: case 128: case 160: case 256:
+    case 512: case 1024: case 2048:
+      return true;
+    default:
+      return false;
+  }
+}
+
+hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s) {
+  if (p.nlines <= 0) return hipSuccess;
+  // the compile-time kernel needs the line axis undistributed on both sides
+  const bool fast = has_fast_line_kernel(p.n) && p.in.ntopo[p.axis] == 1 && p.out.ntopo[p.axis] == 1;
+  if (fast) return fp64 ? dispatch<double>(p, s) : dispatch<float>(p, s);
+  return fp64 ? launch_generic<double>(p, s) : launch_generic<float>(p, s);
+}
+
+// ---------------------------------------------------------------------------
+// Remap (one-GPU exchange / local transpose)
+// ---------------------------------------------------------------------------
+template <typename E>
+__global__ void __launch_bounds__(256) remap_kernel(const RemapArgs a) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < a.total; g += stride) {
+    int32_t idx[kMaxDim] = {0, 0, 0, 0};
+    uint32_t rem = static_cast<uint32_t>(g);
+#pragma unroll
+    for (int k = kMaxDim - 1; k >= 0; --k) {
+      if (k < a.dst.ndim) {
+        uint32_t q = a.iter_div[k].div(rem);
+        idx[a.iter_axis[k]] = static_cast<int32_t>(rem - q * static_cast<uint32_t>(a.iter_ext[k]));
+        rem = q;
+      }
+    }
+    int64_t st;
+    const E* sp = reinterpret_cast<const E*>(element_ptr(a.src, idx, sizeof(E), 0, st));
+    E* dp = reinterpret_cast<E*>(element_ptr(a.dst, idx, sizeof(E), 0, st));
+    *dp = *sp;
+  }
+}
+
+hipError_t launch_remap(const RemapArgs& a, int esize, hipStream_t s) {
+  if (a.total <= 0) return hipSuccess;
+  int64_t blocks = (a.total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  switch (esize) {
+    case 4: hipLaunchKernelGGL(remap_kernel<uint32_t>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(remap_kernel<uint2>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 16: hipLaunchKernelGGL(remap_kernel<uint4>, dim3(blocks), dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Pack / unpack exactly as KokkosFFT_Distributed_PackUnpack.hpp:217-233 and
+// :385-417 (LayoutRight buffer (P, e_0..e_{D-1}), logical axis order).
+// ---------------------------------------------------------------------------
+template <typename E, bool UNPACK>
+__global__ void __launch_bounds__(256) pack_kernel(const PackArgs a) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < a.total; g += stride) {
+    int64_t j[kMaxDim + 1];
+    int64_t rem = g;
+    for (int k = a.ndim; k >= 0; --k) {
+      j[k] = rem % a.buf_ext[k];
+      rem /= a.buf_ext[k];
+    }
+    const int64_t p = j[0];
+    const int64_t base = a.merged_size / a.nprocs, r = a.merged_size % a.nprocs;
+    const int64_t len = base + (p < r ? 1 : 0);
+    const int64_t start = p * base + (p < r ? p : r);
+    int64_t l[kMaxDim];
+    bool ok = true;
+    for (int i = 0; i < a.ndim; ++i) {
+      if (i == a.axis) {
+        if (j[i + 1] >= len) ok = false;
+        l[i] = j[i + 1] + start;
+      } else {
+        l[i] = j[i + 1];
+      }
+    }
+    int64_t off = 0;
+    for (int k = 0; k < a.ndim; ++k) {
+      const int64_t v = l[a.map[k]];
+      if (v >= a.local_ext[k]) ok = false;
+      off = off * a.local_ext[k] + v;
+    }
+    if constexpr (UNPACK) {
+      if (ok) reinterpret_cast<E*>(a.dst)[off] = reinterpret_cast<const E*>(a.src)[g];
+    } else {
+      E v{};
+      if (ok) v = reinterpret_cast<const E*>(a.src)[off];
+      reinterpret_cast<E*>(a.dst)[g] = v;
+    }
+  }
+}
+
+hipError_t launch_pack(const PackArgs& a, int esize, bool unpack, hipStream_t s) {
+  if (a.total <= 0) return hipSuccess;
+  int64_t blocks = (a.total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+#define DFFT_PACK_CASE(T)                                                                   \
+  if (unpack) hipLaunchKernelGGL((pack_kernel<T, true>), dim3(blocks), dim3(256), 0, s, a); \
+  else hipLaunchKernelGGL((pack_kernel<T, false>), dim3(blocks), dim3(256), 0, s, a);
+  switch (esize) {
+    case 4: DFFT_PACK_CASE(uint32_t) break;
+    case 8: DFFT_PACK_CASE(uint2) break;
+    case 16: DFFT_PACK_CASE(uint4) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef DFFT_PACK_CASE
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) all2all_kernel(const All2AllArgs a, int64_t words_per_block) {
+  const int64_t total = static_cast<int64_t>(a.nprocs) * a.nprocs * words_per_block;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const int64_t w = g % words_per_block;
+    const int64_t pq = g / words_per_block;
+    const int q = static_cast<int>(pq / a.nprocs);  // receiving rank
+    const int p = static_cast<int>(pq % a.nprocs);  // sending rank
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.send[p] + q * a.block_bytes);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(a.recv[q] + p * a.block_bytes);
+    dst[w] = src[w];
+  }
+}
+
+hipError_t launch_all2all(const All2AllArgs& a, hipStream_t s) {
+  if (a.block_bytes % 4 != 0) return hipErrorInvalidValue;
+  const int64_t words = a.block_bytes / 4;
+  const int64_t total = static_cast<int64_t>(a.nprocs) * a.nprocs * words;
+  if (total == 0) return hipSuccess;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(all2all_kernel, dim3(blocks), dim3(256), 0, s, a, words);
+  return hipGetLastError();
+}
+
+}  // namespace dfft

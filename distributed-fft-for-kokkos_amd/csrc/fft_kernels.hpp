@@ -1,0 +1,74 @@
+// Host-visible launch interface of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace dfft {
+
+enum LineKind : int32_t { kC2C = 0, kR2C = 1, kC2R = 2 };
+
+// One batched 1-D transform pass along logical axis `axis`: every line of the
+// index space (all axes but `axis`) is read through descriptor `in`,
+// transformed and written through `out`.  The two descriptors may describe
+// different distributions (that is how the exchange between topologies is
+// fused into the first and last pass of a plan).
+struct LinePass {
+  DDesc in, out;
+  int32_t axis;
+  int32_t nbatch;                       // ndim - 1
+  int32_t batch_axis[kMaxDim - 1];      // slowest ... fastest
+  int32_t batch_ext[kMaxDim - 1];
+  FastDiv batch_div[kMaxDim - 1];
+  int32_t nlines;
+  int32_t kind;                         // LineKind
+  int32_t n;                            // transform length (real length for R2C/C2R)
+  int32_t inverse;                      // 1: exponent +2 pi i jk/n
+  int32_t lines_fast;                   // 1: consecutive lanes walk across lines
+  int32_t lines_per_block;
+  const void* tw;                       // exp(-2 pi i e / n), e in [0, n), Real precision
+  double scale;
+};
+
+// Element-wise permuted copy between two distributions of the same logical
+// array (one-GPU form of pack -> all-to-all -> unpack, and of local
+// transposes).  Iterates the destination in its storage order.
+struct RemapArgs {
+  DDesc src, dst;
+  int64_t total;
+  int32_t iter_axis[kMaxDim];  // slowest ... fastest
+  int32_t iter_ext[kMaxDim];
+  FastDiv iter_div[kMaxDim];
+};
+
+bool has_fast_line_kernel(int n);
+hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
+hipError_t launch_remap(const RemapArgs& a, int esize, hipStream_t s);
+
+// Reference-exact pack/unpack of one rank's local array (mapped by `map`,
+// LayoutRight) into/out of a (P, e_0..e_{D-1}) buffer.
+struct PackArgs {
+  int32_t ndim;
+  int32_t nprocs;
+  int32_t axis;           // logical axis split (pack) / merged (unpack)
+  int32_t merged_size;    // local extent of that axis
+  int32_t map[kMaxDim];
+  int64_t local_ext[kMaxDim];   // storage extents of the local array
+  int64_t buf_ext[kMaxDim + 1]; // (P, e_0..e_{D-1})
+  int64_t total;                // elements of the buffer
+  const char* src;
+  char* dst;
+};
+hipError_t launch_pack(const PackArgs& a, int esize, bool unpack, hipStream_t s);
+
+// Device-local all-to-all of P buffers of shape (P, block): recv[q][p] = send[p][q].
+struct All2AllArgs {
+  int32_t nprocs;
+  int64_t block_bytes;
+  const char* send[kMaxRanks];
+  char* recv[kMaxRanks];
+};
+hipError_t launch_all2all(const All2AllArgs& a, hipStream_t s);
+
+}  // namespace dfft

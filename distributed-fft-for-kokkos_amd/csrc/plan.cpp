@@ -1,0 +1,600 @@
+// Plan runtime.  Two strategies execute the same plan:
+//
+//  * fused (default): the transform is a sequence of batched 1-D line passes
+//    over the global index space.  The first pass gathers lines straight out
+//    of the P input-rank buffers, the last pass scatters them into the P
+//    output-rank buffers, and the passes in between work on one plan-owned
+//    global array.  The reference's pack -> MPI_Alltoall -> unpack of every
+//    transpose block (TransBlock.hpp:76-107) therefore costs no extra sweep:
+//    it is the addressing of the first and last pass.
+//  * blockwalk: the reference's block sequence (SlabPlan.hpp:1143-1267,
+//    PencilPlan.hpp:1444-1598): FFT blocks on each rank's mapped local array
+//    and one exchange remap per transpose block.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dfft {
+
+namespace {
+void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(DFFT_ERROR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Vec to_vec(const size_t* a, std::size_t n) { return Vec(a, a + n); }
+
+bool contains(const Vec& v, std::size_t x) { return std::find(v.begin(), v.end(), x) != v.end(); }
+}  // namespace
+
+Plan::Plan(const dfft_desc& d) : d_(d) {
+  if (d.rank < 2 || d.rank > DFFT_MAX_RANK)
+    throw Error(DFFT_ERROR_UNSUPPORTED, "view rank must be in [2, 4]");
+  if (d.fft_rank < 1 || d.fft_rank > 3 || d.fft_rank > d.rank)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "Plan: the Rank of FFT axes must be between 1 and 3");
+  if (d.transform != DFFT_C2C && d.transform != DFFT_R2C)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "transform must be DFFT_C2C or DFFT_R2C");
+  if (d.precision != DFFT_FP32 && d.precision != DFFT_FP64)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "precision must be DFFT_FP32 or DFFT_FP64");
+  if (d.norm < DFFT_NORM_BACKWARD || d.norm > DFFT_NORM_NONE)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "invalid normalization");
+  if (d.exec_mode != DFFT_EXEC_FUSED && d.exec_mode != DFFT_EXEC_BLOCKWALK)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "invalid execution mode");
+  rank_ = static_cast<std::size_t>(d.rank);
+  real_ = d.transform == DFFT_R2C;
+  fp64_ = d.precision == DFFT_FP64;
+  view_layout_ = d.layout == DFFT_LAYOUT_LEFT ? Layout::Left : Layout::Right;
+  in_tl_ = d.in_topology_layout == DFFT_LAYOUT_LEFT ? Layout::Left : Layout::Right;
+  out_tl_ = d.out_topology_layout == DFFT_LAYOUT_LEFT ? Layout::Left : Layout::Right;
+  gin_ = to_vec(d.in_extents, rank_);
+  gout_ = to_vec(d.out_extents, rank_);
+  in_topo_ = to_vec(d.in_topology, rank_);
+  out_topo_ = to_vec(d.out_topology, rank_);
+  axes_ = convert_negative_axes(std::vector<int>(d.axes, d.axes + d.fft_rank), rank_);
+
+  // internal_plan_factory (Plan.hpp:46-113)
+  if (total_size(in_topo_) != total_size(out_topo_))
+    throw Error(DFFT_ERROR_INVALID_TOPOLOGY, "in_topology and out_topology must have the same size.");
+  if (static_cast<int>(total_size(in_topo_)) != d.nranks)
+    throw Error(DFFT_ERROR_INVALID_TOPOLOGY, "topology size must be identical to mpi size.");
+  P_ = static_cast<std::size_t>(d.nranks);
+  if (P_ > static_cast<std::size_t>(kMaxRanks))
+    throw Error(DFFT_ERROR_UNSUPPORTED, "more than 64 virtual ranks per plan");
+  are_valid_extents(gin_, gout_, axes_, real_, false, view_layout_);
+  for (std::size_t a = 0; a < rank_; ++a)
+    if (gin_[a] >= (1u << 30) || gout_[a] >= (1u << 30))
+      throw Error(DFFT_ERROR_UNSUPPORTED, "extent too large");
+
+  kind_ = get_common_topology_type(in_topo_, out_topo_);
+  switch (kind_) {
+    case TopologyType::Shared: name_ = "SharedPlan"; break;
+    case TopologyType::Slab: name_ = "SlabPlan"; break;
+    case TopologyType::Pencil:
+      if (rank_ < 3) throw Error(DFFT_ERROR_UNSUPPORTED, "Pencil plan supports 3D or higher dimensional Views");
+      name_ = "PencilPlan";
+      break;
+    case TopologyType::Empty:
+      throw Error(DFFT_ERROR_UNSUPPORTED, "Empty topology is not supported for FFT plan.");
+    case TopologyType::Brick:
+      throw Error(DFFT_ERROR_UNSUPPORTED, "Brick topology is not supported for FFT plan.");
+    default:
+      throw Error(DFFT_ERROR_INVALID_TOPOLOGY, "Unsupported topology for FFT plan.");
+  }
+
+  Vec id = identity_map(rank_);
+  for (std::size_t r = 0; r < P_; ++r) {
+    in_local_.push_back(compute_local_extents(gin_, in_topo_, id, r, in_tl_));
+    out_local_.push_back(compute_local_extents(gout_, out_topo_, id, r, out_tl_));
+  }
+  fft_extents_ = compute_fft_extents(gin_, gout_, axes_);
+
+  if (kind_ == TopologyType::Slab || kind_ == TopologyType::Pencil) {
+    for (std::size_t r = 0; r < P_; ++r) {
+      analyses_.push_back(block_analyses(kind_, real_, Layout::Right, in_local_[r], gin_, gout_, in_topo_,
+                                         out_topo_, axes_, r, in_tl_, out_tl_));
+      max_buffer_size_ = std::max(max_buffer_size_, analyses_.back().max_buffer_size);
+    }
+    const std::size_t nb = analyses_[0].blocks.size();
+    const std::size_t maxb = kind_ == TopologyType::Slab ? 5 : 8;  // SlabPlan.hpp:905, PencilPlan.hpp:1106
+    if (nb < 1 || nb > maxb)
+      throw Error(DFFT_ERROR_INVALID_TOPOLOGY, "Number blocks must be in [1, " + std::to_string(maxb) + "]");
+  }
+  if (d.exec_mode == DFFT_EXEC_BLOCKWALK) {
+    if (view_layout_ != Layout::Right)
+      throw Error(DFFT_ERROR_UNSUPPORTED, "block-walk execution covers LayoutRight views");
+    if (kind_ == TopologyType::Shared)
+      throw Error(DFFT_ERROR_UNSUPPORTED, "block-walk execution needs a slab or pencil plan");
+  }
+
+  check_hip(hipGetDevice(&device_), "hipGetDevice");
+  // twiddle tables exp(-2 pi i e / n), evaluated in long double, rounded once
+  for (std::size_t k = 0; k < axes_.size(); ++k) {
+    const int n = static_cast<int>(fft_extents_[k]);
+    if (tw_.count(n)) continue;
+    const std::size_t es = fp64_ ? 16 : 8;
+    std::vector<unsigned char> host(es * n);
+    for (int e = 0; e < n; ++e) {
+      const long double a = 6.283185307179586476925286766559005768L * e / n;
+      const long double c = cosl(a), s = -sinl(a);
+      if (fp64_) {
+        double v[2] = {static_cast<double>(c), static_cast<double>(s)};
+        std::memcpy(&host[es * e], v, es);
+      } else {
+        float v[2] = {static_cast<float>(c), static_cast<float>(s)};
+        std::memcpy(&host[es * e], v, es);
+      }
+    }
+    void* p = nullptr;
+    hipError_t err = hipMalloc(&p, std::max<std::size_t>(host.size(), 16));
+    if (err != hipSuccess) throw Error(DFFT_ERROR_OUT_OF_MEMORY, "hipMalloc of twiddle table failed");
+    check_hip(hipMemcpy(p, host.data(), host.size(), hipMemcpyHostToDevice), "twiddle upload");
+    tw_[n] = p;
+    workspace_bytes_ += host.size();
+  }
+
+  const std::size_t cs = fp64_ ? 16 : 8;
+  if (d.exec_mode == DFFT_EXEC_FUSED) {
+    plan_fused();
+    const std::size_t nscratch = total_size(gout_);
+    if (nscratch > 0) {
+      hipError_t err = hipMalloc(&scratch_, nscratch * cs);
+      if (err != hipSuccess) throw Error(DFFT_ERROR_OUT_OF_MEMORY, "hipMalloc of plan scratch failed");
+      workspace_bytes_ += nscratch * cs;
+    }
+  } else {
+    const std::size_t bytes = std::max<std::size_t>(max_buffer_size_ * (cs / 2), 16);
+    for (int k = 0; k < 2; ++k) {
+      for (std::size_t r = 0; r < P_; ++r) {
+        void* p = nullptr;
+        hipError_t err = hipMalloc(&p, bytes);
+        if (err != hipSuccess) throw Error(DFFT_ERROR_OUT_OF_MEMORY, "hipMalloc of exchange buffer failed");
+        walk_bufs_[k].push_back(p);
+        workspace_bytes_ += bytes;
+      }
+    }
+  }
+}
+
+Plan::~Plan() {
+  for (auto& kv : tw_) (void)hipFree(kv.second);
+  if (scratch_) (void)hipFree(scratch_);
+  for (auto& v : walk_bufs_)
+    for (void* p : v) (void)hipFree(p);
+}
+
+void Plan::local_starts(std::size_t r, int which, Vec& ext, Vec& st) const {
+  if (r >= P_) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "rank must be less than topology size.");
+  if (which == 0) compute_local_extents_and_starts(gin_, in_topo_, r, in_tl_, ext, st);
+  else compute_local_extents_and_starts(gout_, out_topo_, r, out_tl_, ext, st);
+}
+
+double Plan::norm_scale(bool forward) const {
+  double n = 1.0;
+  for (auto e : fft_extents_) n *= static_cast<double>(e);
+  switch (d_.norm) {
+    case DFFT_NORM_BACKWARD: return forward ? 1.0 : 1.0 / n;
+    case DFFT_NORM_FORWARD: return forward ? 1.0 / n : 1.0;
+    case DFFT_NORM_ORTHO: return 1.0 / std::sqrt(n);
+    default: return 1.0;
+  }
+}
+
+const void* Plan::twiddles(int n) const { return tw_.at(n); }
+
+// Order of the line passes.  The first pass reads the caller's input
+// distribution and the last writes the output distribution; a pass whose line
+// axis is split across ranks on either side falls back to per-element
+// addressing, so the order prefers axes that are whole on the side they touch.
+void Plan::plan_fused() {
+  const std::size_t last_axis = axes_.back();
+  auto build = [&](const Vec& first_topo, const Vec& last_topo, bool forward) {
+    std::vector<PassSpec> out;
+    Vec rest = axes_;
+    std::size_t first, last = rank_;
+    bool have_last = false;
+    if (real_ && forward) {
+      first = last_axis;
+    } else {
+      first = last_axis;
+      if (!(real_ && !forward && axes_.size() == 1)) {
+        for (auto it = axes_.rbegin(); it != axes_.rend(); ++it) {
+          if (real_ && !forward && *it == last_axis) continue;  // C2R must come last
+          if (first_topo[*it] == 1) {
+            first = *it;
+            break;
+          }
+        }
+        if (real_ && !forward && first == last_axis) {
+          for (auto it = axes_.rbegin(); it != axes_.rend(); ++it)
+            if (*it != last_axis) {
+              first = *it;
+              break;
+            }
+        }
+      }
+    }
+    rest.erase(std::find(rest.begin(), rest.end(), first));
+    if (!rest.empty()) {
+      have_last = true;
+      if (real_ && !forward) {
+        last = last_axis;
+      } else {
+        last = rest.front();
+        for (auto a : rest)
+          if (last_topo[a] == 1) {
+            last = a;
+            break;
+          }
+      }
+      rest.erase(std::find(rest.begin(), rest.end(), last));
+    }
+    std::vector<std::size_t> order{first};
+    for (auto it = rest.rbegin(); it != rest.rend(); ++it) order.push_back(*it);
+    if (have_last) order.push_back(last);
+    for (auto a : order) {
+      PassSpec s;
+      s.axis = static_cast<int>(a);
+      s.n = static_cast<int>(std::max(gin_[a], gout_[a]));
+      s.kind = kC2C;
+      if (real_ && a == last_axis) s.kind = forward ? kR2C : kC2R;
+      out.push_back(s);
+    }
+    return out;
+  };
+  fwd_ = build(in_topo_, out_topo_, true);
+  bwd_ = build(out_topo_, in_topo_, false);
+}
+
+DDesc Plan::user_desc(const Vec& ext, const Vec& topo, Layout tl, void* const* ptrs) const {
+  DDesc d{};
+  d.ndim = static_cast<int32_t>(rank_);
+  d.layout_left = view_layout_ == Layout::Left ? 1 : 0;
+  std::size_t stride = 1;
+  // rank = sum coord[a]*rstride[a], matching rank_to_coord (MPI_Extents.hpp:492-529)
+  if (tl == Layout::Right) {
+    for (std::size_t a = rank_; a-- > 0;) {
+      d.rstride[a] = static_cast<int32_t>(stride);
+      stride *= topo[a];
+    }
+  } else {
+    for (std::size_t a = 0; a < rank_; ++a) {
+      d.rstride[a] = static_cast<int32_t>(stride);
+      stride *= topo[a];
+    }
+  }
+  for (std::size_t a = 0; a < rank_; ++a) {
+    d.ext[a] = static_cast<int32_t>(ext[a]);
+    d.ntopo[a] = static_cast<int32_t>(topo[a]);
+    d.base[a] = static_cast<int32_t>(ext[a] / topo[a]);
+    d.rem[a] = static_cast<int32_t>(ext[a] % topo[a]);
+    d.map[a] = static_cast<int32_t>(a);
+    d.div_big[a] = FastDiv(static_cast<uint32_t>(d.base[a] + 1));
+    d.div_small[a] = FastDiv(static_cast<uint32_t>(d.base[a]));
+  }
+  for (std::size_t r = 0; r < P_; ++r) d.ptr[r] = static_cast<char*>(ptrs[r]);
+  return d;
+}
+
+DDesc Plan::scratch_desc(const Vec& ext, void* p) const {
+  DDesc d{};
+  d.ndim = static_cast<int32_t>(rank_);
+  d.layout_left = view_layout_ == Layout::Left ? 1 : 0;
+  for (std::size_t a = 0; a < rank_; ++a) {
+    d.ext[a] = static_cast<int32_t>(ext[a]);
+    d.ntopo[a] = 1;
+    d.rstride[a] = 0;
+    d.base[a] = static_cast<int32_t>(ext[a]);
+    d.rem[a] = 0;
+    d.map[a] = static_cast<int32_t>(a);
+    d.div_big[a] = FastDiv(static_cast<uint32_t>(ext[a] + 1));
+    d.div_small[a] = FastDiv(static_cast<uint32_t>(ext[a]));
+  }
+  d.ptr[0] = static_cast<char*>(p);
+  return d;
+}
+
+LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, bool inverse,
+                         double scale) const {
+  LinePass p{};
+  p.in = in;
+  p.out = out;
+  p.axis = s.axis;
+  p.kind = s.kind;
+  p.n = s.n;
+  p.inverse = inverse ? 1 : 0;
+  p.scale = scale;
+  p.tw = twiddles(s.n);
+  // batch axes ordered slowest -> fastest in storage order
+  std::vector<int> order;
+  if (in.layout_left) {
+    for (int a = static_cast<int>(rank_) - 1; a >= 0; --a) order.push_back(in.map[a]);
+  } else {
+    for (std::size_t k = 0; k < rank_; ++k) order.push_back(in.map[k]);
+  }
+  const int innermost = order.back();
+  p.nbatch = 0;
+  int64_t nl = 1;
+  for (int a : order) {
+    if (a == s.axis) continue;
+    p.batch_axis[p.nbatch] = a;
+    p.batch_ext[p.nbatch] = in.ext[a];
+    p.batch_div[p.nbatch] = FastDiv(static_cast<uint32_t>(in.ext[a]));
+    nl *= in.ext[a];
+    ++p.nbatch;
+  }
+  if (nl >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many lines in one pass");
+  p.nlines = static_cast<int32_t>(nl);
+  p.lines_fast = s.axis != innermost ? 1 : 0;
+  return p;
+}
+
+void Plan::run_fused(void* const* in, void* const* out, bool forward, hipStream_t stream) {
+  const auto& passes = forward ? fwd_ : bwd_;
+  const double scale = norm_scale(forward);
+  // first and last descriptors
+  DDesc src = forward ? user_desc(gin_, in_topo_, in_tl_, in) : user_desc(gout_, out_topo_, out_tl_, in);
+  DDesc dst = forward ? user_desc(gout_, out_topo_, out_tl_, out) : user_desc(gin_, in_topo_, in_tl_, out);
+  // the scratch array holds the complex spectrum shape (gout)
+  DDesc scr = scratch_desc(gout_, scratch_);
+  // alias check: a single pass may not read and write the same memory
+  bool alias = false;
+  if (passes.size() == 1) {
+    for (std::size_t r = 0; r < P_ && !alias; ++r)
+      for (std::size_t q = 0; q < P_; ++q)
+        if (in[r] != nullptr && in[r] == out[q]) alias = true;
+  }
+  for (std::size_t i = 0; i < passes.size(); ++i) {
+    const bool first = i == 0, last = i + 1 == passes.size();
+    DDesc a = first ? src : scr;
+    DDesc b = last ? (alias ? scratch_desc(forward ? gout_ : gin_, scratch_) : dst) : scr;
+    LinePass p = make_pass(passes[i], a, b, !forward, last ? scale : 1.0);
+    mark(stream);
+    check_hip(launch_line_pass(p, fp64_, stream), "line pass");
+    mark(stream);
+  }
+  if (alias) {
+    RemapArgs ra{};
+    ra.src = scratch_desc(forward ? gout_ : gin_, scratch_);
+    ra.dst = dst;
+    ra.total = 1;
+    for (std::size_t k = 0; k < rank_; ++k) {
+      ra.iter_axis[k] = static_cast<int32_t>(k);
+      ra.iter_ext[k] = dst.ext[k];
+      ra.iter_div[k] = FastDiv(static_cast<uint32_t>(dst.ext[k]));
+      ra.total *= dst.ext[k];
+    }
+    const int es = (forward || !real_) ? (fp64_ ? 16 : 8) : (fp64_ ? 8 : 4);
+    mark(stream);
+    check_hip(launch_remap(ra, es, stream), "remap");
+    mark(stream);
+  }
+}
+
+void Plan::execute(void* const* in, void* const* out, int direction, hipStream_t stream) {
+  if (in == nullptr || out == nullptr) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null pointer table");
+  if (direction != DFFT_FORWARD && direction != DFFT_BACKWARD)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "Invalid FFT direction specified.");
+  const bool forward = direction == DFFT_FORWARD;
+  for (std::size_t r = 0; r < P_; ++r) {
+    const Vec& ie = forward ? in_local_[r] : out_local_[r];
+    const Vec& oe = forward ? out_local_[r] : in_local_[r];
+    if (total_size(ie) > 0 && in[r] == nullptr) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null input pointer");
+    if (total_size(oe) > 0 && out[r] == nullptr) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null output pointer");
+  }
+  check_hip(hipSetDevice(device_), "hipSetDevice");
+  if (total_size(gin_) == 0) return;
+  if (d_.exec_mode == DFFT_EXEC_BLOCKWALK) run_blockwalk(in, out, forward, stream);
+  else run_fused(in, out, forward, stream);
+}
+
+void Plan::mark(hipStream_t s) {
+  if (!timing_) return;
+  hipEvent_t e;
+  check_hip(hipEventCreate(&e), "hipEventCreate");
+  timing_->push_back(e);
+  check_hip(hipEventRecord(e, s), "hipEventRecord");
+}
+
+std::vector<float> Plan::execute_timed(void* const* in, void* const* out, int direction, hipStream_t stream) {
+  std::vector<hipEvent_t> ev;
+  timing_ = &ev;
+  std::vector<float> ms;
+  try {
+    execute(in, out, direction, stream);
+  } catch (...) {
+    timing_ = nullptr;
+    for (auto e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  timing_ = nullptr;
+  hipError_t err = hipStreamSynchronize(stream);
+  for (std::size_t i = 0; err == hipSuccess && i + 1 < ev.size(); i += 2) {
+    float t = 0.f;
+    err = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+    ms.push_back(t);
+  }
+  for (auto e : ev) (void)hipEventDestroy(e);
+  check_hip(err, "timed execute");
+  return ms;
+}
+
+int Plan::num_passes(int direction) const {
+  if (d_.exec_mode == DFFT_EXEC_FUSED) {
+    const auto& p = direction == DFFT_FORWARD ? fwd_ : bwd_;
+    return static_cast<int>(p.size());
+  }
+  return -1;  // data dependent for the block walk; execute_timed reports it
+}
+
+// ---------------------------------------------------------------------------
+// Block walk: SlabInternalPlan/PencilInternalPlan forward_impl/backward_impl.
+// ---------------------------------------------------------------------------
+void Plan::run_blockwalk(void* const* in, void* const* out, bool forward, hipStream_t stream) {
+  const auto& blocks = analyses_[0].blocks;
+  const auto& topos = analyses_[0].topologies;
+  const auto& lays = analyses_[0].layouts;
+  const std::size_t nb = blocks.size();
+  const std::size_t cs = fp64_ ? 16 : 8, rs = fp64_ ? 8 : 4;
+
+  struct State {
+    std::vector<void*> ptr;
+    Vec map, topo, ext;
+    Layout tl;
+    bool is_real;
+  };
+  auto desc_of = [&](const State& s) {
+    DDesc d = user_desc(s.ext, s.topo, s.tl, s.ptr.data());
+    for (std::size_t a = 0; a < rank_; ++a) d.map[a] = static_cast<int32_t>(s.map[a]);
+    return d;
+  };
+  auto remap = [&](const State& from, State& to) {
+    RemapArgs ra{};
+    ra.src = desc_of(from);
+    ra.dst = desc_of(to);
+    ra.total = 1;
+    for (std::size_t k = 0; k < rank_; ++k) {
+      const int a = static_cast<int>(to.map[k]);
+      ra.iter_axis[k] = a;
+      ra.iter_ext[k] = static_cast<int32_t>(to.ext[a]);
+      ra.iter_div[k] = FastDiv(static_cast<uint32_t>(to.ext[a]));
+      ra.total *= static_cast<int64_t>(to.ext[a]);
+    }
+    mark(stream);
+    check_hip(launch_remap(ra, from.is_real ? static_cast<int>(rs) : static_cast<int>(cs), stream), "remap");
+    mark(stream);
+  };
+  int pp = 0;  // ping-pong selector of the next free buffer set
+  auto fresh = [&](const Vec& map, const Vec& topo, const Vec& ext, Layout tl, bool is_real) {
+    State s;
+    s.ptr.assign(walk_bufs_[pp].begin(), walk_bufs_[pp].end());
+    pp ^= 1;
+    s.map = map;
+    s.topo = topo;
+    s.ext = ext;
+    s.tl = tl;
+    s.is_real = is_real;
+    return s;
+  };
+  auto topo_layout = [&](std::size_t ti) {
+    if (kind_ == TopologyType::Slab) return Layout::Right;
+    return lays.at(ti) ? Layout::Right : Layout::Left;
+  };
+  // number of transpose blocks before block bi = index of the topology in use
+  auto topo_index = [&](std::size_t bi) {
+    std::size_t c = 0;
+    for (std::size_t k = 0; k < bi; ++k)
+      if (blocks[k].block_type == BlockType::Transpose) ++c;
+    return c;
+  };
+  double scale = norm_scale(forward);
+  bool scaled = false;
+  Vec ident = identity_map(rank_);
+
+  auto fft_block = [&](State& st, const BlockInfo& b, bool fwd) {
+    // axes of the block: the innermost fft_dim storage axes of the mapped array
+    Vec ax(b.in_map.end() - static_cast<long>(b.fft_dim), b.in_map.end());
+    const bool r2c = real_ && b.block_idx == 0;
+    std::vector<std::size_t> order;
+    if (fwd) {
+      for (auto it = ax.rbegin(); it != ax.rend(); ++it) order.push_back(*it);
+    } else {
+      for (auto a : ax) order.push_back(a);
+    }
+    for (std::size_t k = 0; k < order.size(); ++k) {
+      const std::size_t a = order[k];
+      PassSpec s;
+      s.axis = static_cast<int>(a);
+      s.n = static_cast<int>(std::max(gin_[a], gout_[a]));
+      s.kind = kC2C;
+      State dst = st;
+      if (r2c && a == ax.back()) {
+        s.kind = fwd ? kR2C : kC2R;
+        Vec next_ext = st.ext;
+        next_ext[a] = fwd ? gout_[a] : gin_[a];
+        dst = fresh(st.map, st.topo, next_ext, st.tl, !fwd);
+      }
+      LinePass p = make_pass(s, desc_of(st), desc_of(dst), !fwd, scaled ? 1.0 : scale);
+      scaled = true;
+      mark(stream);
+      check_hip(launch_line_pass(p, fp64_, stream), "line pass");
+      mark(stream);
+      st = dst;
+    }
+  };
+
+  if (forward) {
+    State st;
+    st.ptr.assign(in, in + P_);
+    st.map = ident;
+    st.topo = in_topo_;
+    st.ext = gin_;
+    st.tl = in_tl_;
+    st.is_real = real_;
+    bool user_input = true;
+    for (std::size_t bi = 0; bi < nb; ++bi) {
+      const BlockInfo& b = blocks[bi];
+      const std::size_t ti = topo_index(bi);
+      if (b.block_type == BlockType::FFT) {
+        if (st.map != b.in_map || user_input) {
+          State t = fresh(b.in_map, st.topo, st.ext, st.tl, st.is_real);
+          remap(st, t);
+          st = t;
+          user_input = false;
+        }
+        fft_block(st, b, true);
+      } else {
+        State t = fresh(b.out_map, topos.at(ti + 1), st.ext, topo_layout(ti + 1), st.is_real);
+        remap(st, t);
+        st = t;
+        user_input = false;
+      }
+    }
+    State o;
+    o.ptr.assign(out, out + P_);
+    o.map = ident;
+    o.topo = out_topo_;
+    o.ext = gout_;
+    o.tl = out_tl_;
+    o.is_real = false;
+    remap(st, o);
+  } else {
+    State st;
+    st.ptr.assign(in, in + P_);
+    st.map = ident;
+    st.topo = out_topo_;
+    st.ext = gout_;
+    st.tl = out_tl_;
+    st.is_real = false;
+    bool user_input = true;
+    for (std::size_t bi = nb; bi-- > 0;) {
+      const BlockInfo& b = blocks[bi];
+      const std::size_t ti = topo_index(bi);
+      if (b.block_type == BlockType::FFT) {
+        if (st.map != b.out_map || user_input) {
+          State t = fresh(b.out_map, st.topo, st.ext, st.tl, st.is_real);
+          remap(st, t);
+          st = t;
+          user_input = false;
+        }
+        fft_block(st, b, false);
+      } else {
+        State t = fresh(b.in_map, topos.at(ti), st.ext, topo_layout(ti), st.is_real);
+        remap(st, t);
+        st = t;
+        user_input = false;
+      }
+    }
+    State o;
+    o.ptr.assign(out, out + P_);
+    o.map = ident;
+    o.topo = in_topo_;
+    o.ext = gin_;
+    o.tl = in_tl_;
+    o.is_real = real_;
+    remap(st, o);
+  }
+  (void)contains;
+}
+
+}  // namespace dfft

@@ -1,0 +1,77 @@
+// Plan runtime: the one-GPU replacement of the reference's SlabPlan /
+// PencilPlan / SharedPlan behind the InternalPlan boundary
+// (KokkosFFT_Distributed_InternalPlan.hpp:22-143).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "dfft.h"
+#include "fft_kernels.hpp"
+#include "planner.hpp"
+
+namespace dfft {
+
+struct PassSpec {
+  int axis;
+  int kind;  // LineKind
+  int n;     // transform length
+};
+
+class Plan {
+ public:
+  explicit Plan(const dfft_desc& d);
+  ~Plan();
+  Plan(const Plan&) = delete;
+  Plan& operator=(const Plan&) = delete;
+
+  void execute(void* const* in, void* const* out, int direction, hipStream_t stream);
+  // execute with a HIP event pair around every launch; durations in ms
+  std::vector<float> execute_timed(void* const* in, void* const* out, int direction, hipStream_t stream);
+  int num_passes(int direction) const;
+
+  const std::string& name() const { return name_; }
+  std::size_t nranks() const { return P_; }
+  const Vec& in_local(std::size_t r) const { return in_local_.at(r); }
+  const Vec& out_local(std::size_t r) const { return out_local_.at(r); }
+  void local_starts(std::size_t r, int which, Vec& ext, Vec& st) const;
+  const std::vector<BlockAnalyses>& analyses() const { return analyses_; }
+  std::size_t max_buffer_size() const { return max_buffer_size_; }
+  std::size_t workspace_bytes() const { return workspace_bytes_; }
+
+ private:
+  DDesc user_desc(const Vec& ext, const Vec& topo, Layout tl, void* const* ptrs) const;
+  DDesc scratch_desc(const Vec& ext, void* p) const;
+  LinePass make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, bool inverse, double scale) const;
+  void plan_fused();
+  void run_fused(void* const* in, void* const* out, bool forward, hipStream_t s);
+  void run_blockwalk(void* const* in, void* const* out, bool forward, hipStream_t s);
+  const void* twiddles(int n) const;
+  double norm_scale(bool forward) const;
+
+  dfft_desc d_{};
+  TopologyType kind_ = TopologyType::Shared;
+  std::string name_;
+  std::size_t P_ = 1;
+  std::size_t rank_ = 3;
+  Vec gin_, gout_, axes_, in_topo_, out_topo_, fft_extents_;
+  Layout in_tl_ = Layout::Right, out_tl_ = Layout::Right, view_layout_ = Layout::Right;
+  bool real_ = false, fp64_ = true;
+  int device_ = 0;
+  std::vector<Vec> in_local_, out_local_;
+  std::vector<BlockAnalyses> analyses_;
+  std::size_t max_buffer_size_ = 0;
+  void* scratch_ = nullptr;
+  std::vector<void*> walk_bufs_[2];  // block-walk ping-pong local arrays per rank
+  std::size_t workspace_bytes_ = 0;
+  std::map<int, void*> tw_;
+  std::vector<PassSpec> fwd_, bwd_;
+  // timing hooks (set only by execute_timed)
+  std::vector<hipEvent_t>* timing_ = nullptr;
+  void mark(hipStream_t s);
+};
+
+}  // namespace dfft

@@ -1,0 +1,174 @@
+/* C ABI of the MI355X distributed-FFT library (libdfft.so).
+ *
+ * One process drives P "virtual ranks" that share one GPU's HBM: every
+ * entry point that takes per-rank data takes a table of P device pointers,
+ * one local array per rank, laid out exactly as the reference's per-rank
+ * Kokkos::View (LayoutRight unless stated).  Only plain C types cross this
+ * boundary; `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *
+ * Every function returns a status (DFFT_SUCCESS = 0).  On failure
+ * dfft_last_error() returns the message of the calling thread's last error;
+ * the codes mirror the reference's std::runtime_error throw sites (cited
+ * per function).
+ */
+#ifndef DFFT_H
+#define DFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define DFFT_SUCCESS 0
+#define DFFT_ERROR_INVALID_TOPOLOGY 1 /* Plan.hpp:62-68, Topologies.hpp throws */
+#define DFFT_ERROR_INVALID_EXTENTS 2  /* MPI_Extents.hpp:322-417 */
+#define DFFT_ERROR_MISMATCH 3         /* InternalPlan.hpp:109-142, Plan.hpp:212-238 */
+#define DFFT_ERROR_ALIAS 4            /* TransBlock.hpp:120-186 */
+#define DFFT_ERROR_UNSUPPORTED 5      /* Plan.hpp:85-112 (Brick/Empty), sizes beyond kernels */
+#define DFFT_ERROR_HIP 6              /* HIP runtime failure */
+#define DFFT_ERROR_INVALID_ARGUMENT 7 /* null pointers, bad enums, rank out of range */
+#define DFFT_ERROR_OUT_OF_MEMORY 8
+
+/* ---- enums ------------------------------------------------------------- */
+#define DFFT_C2C 0 /* complex -> complex */
+#define DFFT_R2C 1 /* real -> complex forward; its backward is C2R */
+
+#define DFFT_FP32 0
+#define DFFT_FP64 1
+
+#define DFFT_LAYOUT_RIGHT 0
+#define DFFT_LAYOUT_LEFT 1
+
+/* KokkosFFT::Normalization; default of the reference Plan is backward (Plan.hpp:159) */
+#define DFFT_NORM_BACKWARD 0
+#define DFFT_NORM_FORWARD 1
+#define DFFT_NORM_ORTHO 2
+#define DFFT_NORM_NONE 3
+
+/* KokkosFFT::Direction */
+#define DFFT_FORWARD 0
+#define DFFT_BACKWARD 1
+
+/* execution strategies of a plan */
+#define DFFT_EXEC_FUSED 0     /* exchange fused into the FFT passes (default) */
+#define DFFT_EXEC_BLOCKWALK 1 /* the reference's block sequence: FFT blocks + exchanges */
+
+#define DFFT_MAX_RANK 4
+#define DFFT_MAX_VRANKS 64
+
+typedef struct dfft_plan dfft_plan;
+
+/* Plan description: what the reference's Plan constructor receives
+ * (Plan.hpp:154-181) with the per-rank views replaced by the global extents
+ * the reference reconstructs with MPI_Allgather (MPI_Extents.hpp:114-129). */
+typedef struct dfft_desc {
+  int rank;                               /* view rank, 2..4 */
+  int fft_rank;                           /* number of transformed axes, 1..3 */
+  int axes[3];                            /* transform axes (negative allowed) */
+  int transform;                          /* DFFT_C2C or DFFT_R2C */
+  int precision;                          /* DFFT_FP32 or DFFT_FP64 */
+  int layout;                             /* view layout, DFFT_LAYOUT_RIGHT */
+  size_t in_extents[DFFT_MAX_RANK];       /* global input extents (real for R2C) */
+  size_t out_extents[DFFT_MAX_RANK];      /* global output extents */
+  size_t in_topology[DFFT_MAX_RANK];
+  size_t out_topology[DFFT_MAX_RANK];
+  int in_topology_layout;                 /* Topology<..., LayoutRight|LayoutLeft> */
+  int out_topology_layout;
+  int norm;                               /* DFFT_NORM_* */
+  int nranks;                             /* P: must equal the topology size */
+  int exec_mode;                          /* DFFT_EXEC_* */
+} dfft_desc;
+
+/* BlockInfo<DIM> (Types.hpp:61-154) as a POD; arrays hold `rank` entries
+ * (`buffer_extents` rank+1). */
+typedef struct dfft_block_info {
+  size_t in_topology[DFFT_MAX_RANK];
+  size_t out_topology[DFFT_MAX_RANK];
+  size_t in_extents[DFFT_MAX_RANK];
+  size_t out_extents[DFFT_MAX_RANK];
+  size_t buffer_extents[DFFT_MAX_RANK + 1];
+  size_t in_map[DFFT_MAX_RANK];
+  size_t out_map[DFFT_MAX_RANK];
+  size_t in_axis, out_axis, fft_dim, comm_axis;
+  int block_type; /* 0 = Transpose, 1 = FFT */
+  size_t block_idx;
+} dfft_block_info;
+
+const char* dfft_last_error(void);
+int dfft_version(int* major, int* minor);
+
+/* ---- plans ------------------------------------------------------------- */
+/* Replaces Plan::Plan + internal_plan_factory (Plan.hpp:46-113, 154-181). */
+int dfft_plan_create(const dfft_desc* desc, dfft_plan** plan);
+/* Replaces the Plan destructor (buffers, sub-plans, PencilPlan.hpp:1190-1198). */
+int dfft_plan_destroy(dfft_plan* plan);
+/* Replaces Plan::name() (Plan.hpp:199): "SharedPlan", "SlabPlan", "PencilPlan". */
+int dfft_plan_name(const dfft_plan* plan, char* buf, size_t len);
+/* Local extents/starts of virtual rank `vrank` for the input (which = 0) or the
+ * output (which = 1): compute_local_extents_and_starts (MPI_Extents.hpp:546-577). */
+int dfft_plan_local_extents(const dfft_plan* plan, int vrank, int which, size_t* extents, size_t* starts);
+/* Block list of virtual rank `vrank` (Slab/PencilBlockAnalysesInternal, BlockAnalyses.hpp:415-506). */
+int dfft_plan_num_blocks(const dfft_plan* plan, int* nblocks);
+int dfft_plan_block_info(const dfft_plan* plan, int vrank, int idx, dfft_block_info* info);
+/* Global maximum of the per-rank buffer sizes in scalar units (m_max_buffer_size). */
+int dfft_plan_max_buffer_size(const dfft_plan* plan, size_t* size);
+/* Device bytes the plan owns (scratch + twiddle tables). */
+int dfft_plan_workspace_bytes(const dfft_plan* plan, size_t* bytes);
+
+/* Replaces execute(plan, in, out, direction) (Plan.hpp:212-238).  Forward
+ * reads `in` (P local input arrays) and writes `out` (P local output arrays);
+ * backward reads `in` (the P complex spectra) and writes `out` (P arrays of
+ * the plan's input shape).  Inputs are not modified. */
+int dfft_execute(dfft_plan* plan, void* const* in, void* const* out, int direction, void* stream);
+
+/* Number of kernel launches (line passes, remaps) one execute of `direction` issues. */
+int dfft_plan_num_passes(const dfft_plan* plan, int direction, int* npasses);
+/* dfft_execute that also records a HIP event pair around every launch on
+ * `stream`, synchronizes, and writes each launch's duration (ms) into
+ * ms_per_pass[0..npasses) -- the per-kernel timing the reference gets from
+ * Kokkos::Profiling regions (MPI_All2All.hpp:29-32). */
+int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int direction, void* stream,
+                       float* ms_per_pass, int capacity, int* npasses);
+
+/* ---- exchange layer (standalone, one GPU) ------------------------------ */
+/* pack() (PackUnpack.hpp:439-460) of one rank's local array `src` (storage
+ * extents src_extents, mapped by src_map, LayoutRight) into the buffer
+ * (nprocs, buffer_extents[0..rank-1]) along logical `axis`. */
+int dfft_pack(const void* src, const size_t* src_extents, const size_t* src_map, int rank, void* buffer,
+              const size_t* buffer_extents, int nprocs, int axis, int elem_size, void* stream);
+/* unpack() (PackUnpack.hpp:478-499): the inverse placement into `dst`. */
+int dfft_unpack(const void* buffer, const size_t* buffer_extents, int nprocs, void* dst,
+                const size_t* dst_extents, const size_t* dst_map, int rank, int axis, int elem_size,
+                void* stream);
+/* all2all() (MPI_All2All.hpp:23-72) among P virtual ranks: block p of rank
+ * r's send buffer lands as block r of rank p's receive buffer. */
+int dfft_all2all(void* const* send, void* const* recv, int nprocs, size_t block_bytes, void* stream);
+/* TransBlock::operator() (TransBlock.hpp:76-107) for all P virtual ranks in
+ * one fused remap: the local arrays `in` (distribution `in_topology`, map
+ * `in_map`) become `out` (distribution `out_topology`, map `out_map`). */
+int dfft_exchange(void* const* in, void* const* out, int rank, const size_t* global_extents,
+                  const size_t* in_topology, int in_topology_layout, const size_t* in_map,
+                  const size_t* out_topology, int out_topology_layout, const size_t* out_map, int elem_size,
+                  void* stream);
+
+/* ---- planner (host only, no device needed) ----------------------------- */
+/* get_all_slab_topologies (Topologies.hpp:420-618); topologies written as
+ * ntopo x rank entries; *ntopo holds the capacity on entry. */
+int dfft_get_all_slab_topologies(int rank, const size_t* in_topology, const size_t* out_topology,
+                                 int fft_rank, const int* axes, size_t* topologies, int* ntopo);
+/* get_all_pencil_topologies (Topologies.hpp:642-811). */
+int dfft_get_all_pencil_topologies(int rank, const size_t* in_topology, int in_layout,
+                                   const size_t* out_topology, int out_layout, int fft_rank, const int* axes,
+                                   size_t* topologies, size_t* trans_axes, size_t* layouts, int* ntopo);
+/* Block analyses of one rank without allocating a plan. */
+int dfft_block_analyses(const dfft_desc* desc, int vrank, dfft_block_info* infos, int* nblocks,
+                        size_t* max_buffer_size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFFT_H */

@@ -1,0 +1,144 @@
+"""GPU parity of the distributed plans against the oracle (through the C ABI).
+
+Mirrors the reference's plan tests (Test_SlabPlan.cpp, Test_PencilPlan.cpp):
+every virtual rank's output must equal the global transform sliced to that
+rank's block of the output topology, and backward(forward(x)) == x.  The
+reference checks float at rtol 1e-5; here FP64 is held to rel-L2 <= 1e-12 and
+FP32 to rel-L2 <= 1e-5 against the FP64 oracle.
+"""
+
+import itertools
+
+import numpy as np
+import pytest
+
+from helpers import gather_rel_l2, reference_spectrum, run_gpu_plan, seeded_input
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def _check(dev, gin, axes, tin, tout, real, fp64=True, norm="backward", exec_mode=0, seed=0, **kw):
+    x = seeded_input(gin, real, seed)
+    outs, backs, ins, plan = run_gpu_plan(dev, x, gin, axes, tin, tout, real, fp64, norm, exec_mode, **kw)
+    ref = reference_spectrum(x, gin, axes, tout, real, norm, kw.get("tout_layout"))
+    tol = TOL64 if fp64 else TOL32
+    e_fwd = gather_rel_l2(outs, ref)
+    e_rt = gather_rel_l2(backs, ins)
+    assert e_fwd <= tol, f"forward rel-L2 {e_fwd:.3e} {gin} {axes} {tin}->{tout} real={real}"
+    assert e_rt <= tol, f"round trip rel-L2 {e_rt:.3e} {gin} {axes} {tin}->{tout} real={real}"
+    return plan
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+@pytest.mark.parametrize("real", [False, True])
+def test_slab_3d_all_pairs_all_orders(dev, P, real):
+    """Test_SlabPlan.cpp:2084-2290 -- (8,7,5), every slab pair, every axis order."""
+    topos = [[1, 1, P], [1, P, 1], [P, 1, 1]]
+    for tin, tout in itertools.product(topos, topos):
+        for axes in itertools.permutations([0, 1, 2]):
+            plan = _check(dev, [8, 7, 5], list(axes), tin, tout, real, seed=P)
+            assert plan.name() == ("SharedPlan" if P == 1 else "SlabPlan")
+
+
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("fp64", [True, False])
+def test_pencil_2x2_and_2x3(dev, real, fp64):
+    """Test_PencilPlan.cpp:3653-... (8,7,6) style, pencil pairs, every axis order."""
+    for px, py in [(2, 2), (2, 3)]:
+        topos = [[1, px, py], [px, 1, py], [px, py, 1]]
+        for tin, tout in itertools.product(topos, topos):
+            for axes in itertools.permutations([0, 1, 2]):
+                plan = _check(dev, [8, 7, 6], list(axes), tin, tout, real, fp64, seed=px + py)
+                assert plan.name() == "PencilPlan"
+
+
+@pytest.mark.parametrize("real", [False, True])
+def test_slab_2d_and_1d_axes(dev, real):
+    """FFT over 1 or 2 axes of a 3-D view, and 2-D views (Test_SlabPlan.cpp:91, :333)."""
+    for P in [2, 3]:
+        topos = [[1, 1, P], [1, P, 1], [P, 1, 1]]
+        for tin, tout in itertools.product(topos, topos):
+            for axes in [[0], [1], [2], [0, 1], [1, 2], [2, 0]]:
+                _check(dev, [8, 7, 9], axes, tin, tout, real, seed=7)
+        for tin, tout in itertools.product([[1, P], [P, 1]], repeat=2):
+            for axes in [[0, 1], [1, 0], [0], [1]]:
+                _check(dev, [8, 7], axes, tin, tout, real, seed=8)
+
+
+@pytest.mark.parametrize("norm", ["backward", "forward", "ortho", "none"])
+def test_normalizations(dev, norm):
+    x = seeded_input([12, 10, 8], False, 3)
+    outs, backs, ins, _ = run_gpu_plan(dev, x, [12, 10, 8], [0, 1, 2], [2, 1, 1], [1, 2, 1], False, True, norm)
+    ref = reference_spectrum(x, [12, 10, 8], [0, 1, 2], [1, 2, 1], False, norm)
+    assert gather_rel_l2(outs, ref) <= TOL64
+    scale = {"backward": 1.0, "forward": 1.0, "ortho": 1.0, "none": 12 * 10 * 8}[norm]
+    assert gather_rel_l2([b / scale for b in backs], ins) <= TOL64
+
+
+def test_4d_batched_views(dev):
+    """4-D views with a leading batch axis (Test_SlabPlan.cpp:2738, NS batched example)."""
+    for real in [False, True]:
+        _check(dev, [3, 8, 7, 6], [1, 2, 3], [1, 2, 1, 1], [1, 1, 2, 1], real, seed=11)
+        _check(dev, [3, 8, 7, 6], [1, 2, 3], [1, 2, 2, 1], [1, 1, 2, 2], real, seed=12)
+
+
+def test_config_a_64cubed_c2c_slab(dev):
+    """64^3 complex128, slab (2,1,1) -> (1,2,1), U[-1,1] seed 1, fwd then bwd."""
+    _check(dev, [64, 64, 64], [0, 1, 2], [2, 1, 1], [1, 2, 1], False, seed=1)
+
+
+def test_config_b_256cubed_r2c_pencil(dev):
+    """256^3 float64 N(0,1) seed 2, pencil (2,2,1) -> (1,2,2): rel-L2 <= 1e-12."""
+    gin = [256, 256, 256]
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(gin)
+    outs, backs, ins, plan = run_gpu_plan(dev, x, gin, [0, 1, 2], [2, 2, 1], [1, 2, 2], True)
+    ref = reference_spectrum(x, gin, [0, 1, 2], [1, 2, 2], True)
+    assert plan.name() == "PencilPlan"
+    assert gather_rel_l2(outs, ref) <= TOL64
+    assert gather_rel_l2(backs, ins) <= TOL64
+
+
+def test_closed_form_single_mode(dev):
+    """sin(3x+5y+7z) on [0,2pi)^3: forward spectrum -i N^3/2 at (3,5,7), +i N^3/2 at -k."""
+    n = 32
+    t = 2 * np.pi * np.arange(n) / n
+    X, Y, Z = np.meshgrid(t, t, t, indexing="ij")
+    f = np.sin(3 * X + 5 * Y + 7 * Z)
+    for fp64 in [True, False]:
+        outs, _, _, plan = run_gpu_plan(dev, f, [n] * 3, [0, 1, 2], [4, 1, 1], [1, 4, 1], False, fp64)
+        from oracle import plan_exec as PE
+        spec = PE.gather(outs, [n] * 3, [1, 4, 1], np.complex128)
+        expect = np.zeros([n] * 3, complex)
+        expect[3, 5, 7] = -0.5j * n ** 3
+        expect[-3, -5, -7] = 0.5j * n ** 3
+        tol = 1e-12 if fp64 else 1e-5
+        assert np.abs(spec - expect).max() / (0.5 * n ** 3) <= tol
+
+
+def test_blockwalk_matches_fused(dev):
+    """The reference block sequence (FFT blocks + exchanges) gives the same result."""
+    for gin, tin, tout, axes, real in [
+        ([8, 7, 5], [2, 1, 1], [1, 2, 1], [0, 1, 2], False),
+        ([8, 7, 5], [1, 1, 3], [3, 1, 1], [2, 0, 1], True),
+        ([8, 7, 6], [1, 2, 2], [2, 1, 2], [0, 1, 2], True),
+        ([8, 7, 6], [2, 2, 1], [1, 2, 2], [1, 0, 2], False),
+        ([8, 7, 6], [1, 2, 3], [2, 3, 1], [0, 1, 2], True),
+    ]:
+        _check(dev, gin, axes, tin, tout, real, exec_mode=1, seed=5)
+
+
+def test_layout_left_topologies(dev):
+    """Topology<..., LayoutLeft> (Test_PencilPlan.cpp:102): rank ordering per layout."""
+    _check(dev, [8, 7, 6], [0, 1, 2], [1, 2, 3], [3, 2, 1], False, tin_layout="right", tout_layout="left")
+    _check(dev, [8, 7, 6], [0, 1, 2], [2, 3, 1], [1, 2, 3], True, tin_layout="left", tout_layout="right")
+
+
+def test_nonsmooth_lengths_generic_kernel(dev):
+    """Prime and mixed lengths go through the generic kernel."""
+    for gin in ([11, 13, 17], [9, 25, 49], [96, 10, 160]):
+        for real in [False, True]:
+            _check(dev, gin, [0, 1, 2], [2, 1, 1], [1, 2, 1], real, seed=9)
