@@ -101,6 +101,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
+                    help="run one precision only (profiling runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the P virtual ranks share one MI355X; runs on more "
@@ -111,8 +115,19 @@ def main():
     import dfft
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8, "launches": 3}))
+        return
     results = {}
-    for tag, cdt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
+    precisions = (("fp64", torch.complex128), ("fp32", torch.complex64))
+    if args.only:
+        precisions = tuple(p for p in precisions if p[0] == args.only)
+    for tag, cdt in precisions:
         log(f"[bench] {tag}: allocating 512^3 slab P=4")
         ins, outs = make_inputs(torch, dfft, dev, cdt, seed=3)
         backs = [torch.empty_like(a) for a in ins]
@@ -128,6 +143,9 @@ def main():
         del plan, ins, outs, backs
         torch.cuda.empty_cache()
 
+    if args.only:
+        print(json.dumps({"only": args.only, **{k: v for k, v in results[args.only].items()}}), flush=True)
+        return
     esz = 16
     B = 2 * N ** 3 * esz  # input + output array of one transform (bytes)
     r64 = results["fp64"]
