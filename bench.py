@@ -101,6 +101,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the plan autotuner")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
                     help="run one precision only (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -132,6 +133,13 @@ def main():
         ins, outs = make_inputs(torch, dfft, dev, cdt, seed=3)
         backs = [torch.empty_like(a) for a in ins]
         plan = dfft.Plan(ins, outs, [0, 1, 2], [P, 1, 1], [1, P, 1])
+        if not args.no_tune:
+            t0 = time.perf_counter()
+            plan.autotune(ins, outs)  # overwrites ins/outs: inputs are regenerated below
+            torch.cuda.synchronize()
+            log(f"[bench] {tag}: autotune {time.perf_counter() - t0:.2f} s -> {plan.options()}")
+            del ins, outs
+            ins, outs = make_inputs(torch, dfft, dev, cdt, seed=3)
         ms = time_roundtrip(torch, dfft, plan, ins, outs, backs, args.steps, args.warmup)
         fw, bw = pass_times(dfft, plan, ins, outs, backs, max(2, args.steps // 2))
         log(f"[bench] {tag}: {ms:.3f} ms per fwd+bwd; passes fwd {fw} bwd {bw}")
@@ -139,7 +147,7 @@ def main():
         err = math.sqrt(sum(float(torch.linalg.vector_norm(b - a)) ** 2 for a, b in zip(ins, backs)) /
                         sum(float(torch.linalg.vector_norm(a)) ** 2 for a in ins))
         results[tag] = {"ms": ms, "fwd_pass_ms": fw, "bwd_pass_ms": bw, "roundtrip_rel_l2": err,
-                        "workspace_bytes": plan.workspace_bytes()}
+                        "workspace_bytes": plan.workspace_bytes(), "options": plan.options()}
         del plan, ins, outs, backs
         torch.cuda.empty_cache()
 

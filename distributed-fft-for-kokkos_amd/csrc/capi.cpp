@@ -178,6 +178,27 @@ int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int d
   });
 }
 
+int dfft_plan_set_option(dfft_plan* plan, const char* key, long value) {
+  return guarded([&] {
+    need(plan && key, "null pointer");
+    plan->impl->set_option(key, value);
+  });
+}
+
+int dfft_plan_get_option(const dfft_plan* plan, const char* key, long* value) {
+  return guarded([&] {
+    need(plan && key && value, "null pointer");
+    *value = plan->impl->get_option(key);
+  });
+}
+
+int dfft_plan_autotune(dfft_plan* plan, void* const* in, void* const* out, void* stream) {
+  return guarded([&] {
+    need(plan && in && out, "null pointer");
+    plan->impl->autotune(in, out, static_cast<hipStream_t>(stream));
+  });
+}
+
 int dfft_pack(const void* src, const size_t* src_extents, const size_t* src_map, int rank, void* buffer,
               const size_t* buffer_extents, int nprocs, int axis, int elem_size, void* stream) {
   return guarded([&] {

@@ -27,7 +27,10 @@ struct FPlan {
 
 constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
 
-constexpr FPlan make_plan(int n) {
+// Radix-8 stages first, then one 4 or 2, then 3s and 5s.  R (values per
+// thread) is the least common multiple of the radices, doubled while below
+// `rmin` and still a divisor of n.
+constexpr FPlan make_plan(int n, int rmin = 1) {
   FPlan p;
   if (n <= 1) return p;
   int m = n;
@@ -39,6 +42,7 @@ constexpr FPlan make_plan(int n) {
   p.ok = (m == 1);
   int R = 1;
   for (int s = 0; s < p.nst; ++s) R = R / cgcd(R, p.rad[s]) * p.rad[s];
+  while (R < rmin && n % (2 * R) == 0) R *= 2;
   p.R = R;
   return p;
 }
@@ -174,7 +178,7 @@ template <typename Real>
 __device__ __forceinline__ void decode_line(const LinePass& p, int32_t line, int32_t* idx) {
 #pragma unroll
   for (int a = 0; a < kMaxDim; ++a) idx[a] = 0;
-  uint32_t rem = static_cast<uint32_t>(line);
+  uint32_t rem = static_cast<uint32_t>(line + p.line_begin);
 #pragma unroll
   for (int j = kMaxDim - 2; j >= 0; --j) {
     if (j < p.nbatch) {
@@ -210,8 +214,8 @@ template <typename Real> struct LdsPad;
 template <> struct LdsPad<double> { static constexpr int shift = 3; };
 template <> struct LdsPad<float> { static constexpr int shift = 4; };
 
-template <typename Real, int N> struct FastCfg {
-  static constexpr FPlan P = make_plan(N);
+template <typename Real, int N, int RM> struct FastCfg {
+  static constexpr FPlan P = make_plan(N, RM);
   static constexpr int R = P.R;
   static constexpr int T = N / R;
   static constexpr int NST = P.nst;
@@ -219,9 +223,31 @@ template <typename Real, int N> struct FastCfg {
   __device__ static __forceinline__ int lidx(int pos) { return pos + (pos >> LdsPad<Real>::shift); }
 };
 
-template <typename Real, int N, int S>
-__device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const cplx_t<Real>* tw) {
-  using Cfg = FastCfg<Real, N>;
+// Twiddles of a radix-r butterfly are the powers w^j (j = 1..r-1) of one
+// table entry w = exp(-2 pi i k / (L r)): one FP64 load, the other powers by
+// FP64 products (at most three multiplications deep), each rounded once to the
+// kernel's precision.
+template <int r>
+__device__ __forceinline__ void twiddle_powers(double2 w, double2* p) {
+  p[1] = w;
+  if constexpr (r > 2) p[2] = cmul(w, w);
+  if constexpr (r > 3) p[3] = cmul(p[2], w);
+  if constexpr (r > 4) p[4] = cmul(p[2], p[2]);
+  if constexpr (r > 5) p[5] = cmul(p[4], w);
+  if constexpr (r > 6) p[6] = cmul(p[3], p[3]);
+  if constexpr (r > 7) p[7] = cmul(p[4], p[3]);
+}
+
+template <typename C> __device__ __forceinline__ C to_prec(double2 w);
+template <> __device__ __forceinline__ double2 to_prec<double2>(double2 w) { return w; }
+template <> __device__ __forceinline__ float2 to_prec<float2>(double2 w) {
+  return float2{static_cast<float>(w.x), static_cast<float>(w.y)};
+}
+
+template <typename Real, int N, int RM, int S>
+__device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const double2* tw) {
+  using Cfg = FastCfg<Real, N, RM>;
+  using C = cplx_t<Real>;
   constexpr int r = Cfg::P.rad[S];
   constexpr int L = span_before(N, S);
   constexpr int R = Cfg::R;
@@ -232,17 +258,19 @@ __device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const cplx
       const int beta = t * NB + q;
       const int k = beta % L;
       constexpr int ts = N / (L * r);
+      double2 pw[8];
+      twiddle_powers<r>(tw[k * ts], pw);
 #pragma unroll
-      for (int j = 1; j < r; ++j) v[q * r + j] = cmul(v[q * r + j], tw[j * k * ts]);
+      for (int j = 1; j < r; ++j) v[q * r + j] = cmul(v[q * r + j], to_prec<C>(pw[j]));
     }
     bfly<r>(v + q * r);
   }
 }
 
 // write the outputs of stage S, read the inputs of stage S+1
-template <typename Real, int N, int S>
+template <typename Real, int N, int RM, int S>
 __device__ __forceinline__ void stage_exchange(cplx_t<Real>* v, int t, cplx_t<Real>* line_lds) {
-  using Cfg = FastCfg<Real, N>;
+  using Cfg = FastCfg<Real, N, RM>;
   constexpr int r = Cfg::P.rad[S];
   constexpr int L = span_before(N, S);
   constexpr int R = Cfg::R;
@@ -267,21 +295,21 @@ __device__ __forceinline__ void stage_exchange(cplx_t<Real>* v, int t, cplx_t<Re
   }
 }
 
-template <typename Real, int N, int S> struct StageLoop {
-  __device__ static __forceinline__ void run(cplx_t<Real>* v, int t, const cplx_t<Real>* tw,
+template <typename Real, int N, int RM, int S> struct StageLoop {
+  __device__ static __forceinline__ void run(cplx_t<Real>* v, int t, const double2* tw,
                                              cplx_t<Real>* line_lds) {
-    stage_compute<Real, N, S>(v, t, tw);
-    if constexpr (S + 1 < FastCfg<Real, N>::NST) {
-      stage_exchange<Real, N, S>(v, t, line_lds);
-      StageLoop<Real, N, S + 1>::run(v, t, tw, line_lds);
+    stage_compute<Real, N, RM, S>(v, t, tw);
+    if constexpr (S + 1 < FastCfg<Real, N, RM>::NST) {
+      stage_exchange<Real, N, RM, S>(v, t, line_lds);
+      StageLoop<Real, N, RM, S + 1>::run(v, t, tw, line_lds);
     }
   }
 };
 
-template <typename Real, int N>
+template <typename Real, int N, int RM>
 __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   using C = cplx_t<Real>;
-  using Cfg = FastCfg<Real, N>;
+  using Cfg = FastCfg<Real, N, RM>;
   constexpr int R = Cfg::R, T = Cfg::T, NST = Cfg::NST;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
@@ -297,7 +325,7 @@ __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   }
   const int32_t line = blockIdx.x * B + b;
   const bool valid = line < p.nlines;
-  const C* tw = reinterpret_cast<const C*>(p.tw);
+  const double2* tw = reinterpret_cast<const double2*>(p.tw);
   C* line_lds = lds + b * Cfg::LDSN;
 
   LineIO<Real> io;
@@ -314,7 +342,7 @@ __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
       for (int j = 0; j < r0; ++j) v[q * r0 + j] = valid ? io.load(beta + j * (N / r0)) : C{Real(0), Real(0)};
     }
   }
-  StageLoop<Real, N, 0>::run(v, t, tw, line_lds);
+  StageLoop<Real, N, RM, 0>::run(v, t, tw, line_lds);
 
   constexpr int rl = Cfg::P.rad[NST - 1];
   if constexpr (rl != R) {
@@ -355,7 +383,7 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
   const int B = p.lines_per_block;
   C* bufA = reinterpret_cast<C*>(smem_raw);
   C* bufB = bufA + B * n;
-  const C* tw = reinterpret_cast<const C*>(p.tw);
+  const double2* tw = reinterpret_cast<const double2*>(p.tw);
   const int32_t total = B * n;
   const int64_t ies = p.kind == kR2C ? sizeof(Real) : sizeof(C);
   const int64_t oes = p.kind == kC2R ? sizeof(Real) : sizeof(C);
@@ -411,7 +439,7 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
       int32_t ex = 0;
       for (int32_t j = 0; j < r; ++j) {
         C x = src[beta + j * nr];
-        acc = cadd(acc, cmul(x, tw[ex * ts]));
+        acc = cadd(acc, cmul(x, to_prec<C>(tw[ex * ts])));
         ex += step;
         if (ex >= Lr) ex -= Lr * (ex / Lr);
       }
@@ -448,31 +476,42 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
 // ---------------------------------------------------------------------------
 namespace {
 
-template <typename Real, int N>
+template <typename Real, int N, int RM>
 hipError_t launch_fast(LinePass p, hipStream_t s) {
-  using Cfg = FastCfg<Real, N>;
+  using Cfg = FastCfg<Real, N, RM>;
   using C = cplx_t<Real>;
   constexpr int T = Cfg::T;
-  int B;
-  if (p.lines_fast) {
-    const int coalesce = 128 / static_cast<int>(sizeof(C));  // lines per 128-byte row segment
-    B = coalesce;
-    while (B * T > 1024 && B > 1) B /= 2;
-  } else {
-    B = T >= 256 ? 1 : 256 / T;
+  int B = p.lines_per_block;
+  if (B <= 0) {
+    if (p.lines_fast) {
+      B = 128 / static_cast<int>(sizeof(C));  // lines per 128-byte row segment
+      while (B * T > 1024 && B > 1) B /= 2;
+    } else {
+      B = T >= 256 ? 1 : 256 / T;
+    }
+    while (B > 1 && static_cast<size_t>(B) * Cfg::LDSN * sizeof(C) > 160 * 1024) B /= 2;
   }
-  while (B > 1 && static_cast<size_t>(B) * Cfg::LDSN * sizeof(C) > 160 * 1024) B /= 2;
-  p.lines_per_block = B;
   const size_t smem = static_cast<size_t>(B) * Cfg::LDSN * sizeof(C);
+  if (B * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
+  p.lines_per_block = B;
   if (smem > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_line_fast<Real, N>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
     if (e != hipSuccess) return e;
   }
   const unsigned grid = static_cast<unsigned>((p.nlines + B - 1) / B);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((fft_line_fast<Real, N>), dim3(grid), dim3(T * B), smem, s, p);
+  hipLaunchKernelGGL((fft_line_fast<Real, N, RM>), dim3(grid), dim3(T * B), smem, s, p);
   return hipGetLastError();
+}
+
+template <typename Real, int N>
+hipError_t launch_fast_r(const LinePass& p, hipStream_t s) {
+  switch (p.values_per_thread) {
+    case 4: return launch_fast<Real, N, 4>(p, s);
+    case 16: return launch_fast<Real, N, 16>(p, s);
+    default: return launch_fast<Real, N, 8>(p, s);
+  }
 }
 
 template <typename Real>
@@ -511,19 +550,19 @@ hipError_t launch_generic(LinePass p, hipStream_t s) {
 template <typename Real>
 hipError_t dispatch(const LinePass& p, hipStream_t s) {
   switch (p.n) {
-    case 2: return launch_fast<Real, 2>(p, s);
-    case 4: return launch_fast<Real, 4>(p, s);
-    case 8: return launch_fast<Real, 8>(p, s);
-    case 16: return launch_fast<Real, 16>(p, s);
-    case 32: return launch_fast<Real, 32>(p, s);
-    case 64: return launch_fast<Real, 64>(p, s);
-    case 96: return launch_fast<Real, 96>(p, s);
-    case 128: return launch_fast<Real, 128>(p, s);
-    case 160: return launch_fast<Real, 160>(p, s);
-    case 256: return launch_fast<Real, 256>(p, s);
-    case 512: return launch_fast<Real, 512>(p, s);
-    case 1024: return launch_fast<Real, 1024>(p, s);
-    case 2048: return launch_fast<Real, 2048>(p, s);
+    case 2: return launch_fast_r<Real, 2>(p, s);
+    case 4: return launch_fast_r<Real, 4>(p, s);
+    case 8: return launch_fast_r<Real, 8>(p, s);
+    case 16: return launch_fast_r<Real, 16>(p, s);
+    case 32: return launch_fast_r<Real, 32>(p, s);
+    case 64: return launch_fast_r<Real, 64>(p, s);
+    case 96: return launch_fast_r<Real, 96>(p, s);
+    case 128: return launch_fast_r<Real, 128>(p, s);
+    case 160: return launch_fast_r<Real, 160>(p, s);
+    case 256: return launch_fast_r<Real, 256>(p, s);
+    case 512: return launch_fast_r<Real, 512>(p, s);
+    case 1024: return launch_fast_r<Real, 1024>(p, s);
+    case 2048: return launch_fast_r<Real, 2048>(p, s);
     default: return launch_generic<Real>(p, s);
   }
 }

@@ -21,12 +21,14 @@ struct LinePass {
   int32_t batch_axis[kMaxDim - 1];      // slowest ... fastest
   int32_t batch_ext[kMaxDim - 1];
   FastDiv batch_div[kMaxDim - 1];
-  int32_t nlines;
+  int32_t nlines;                       // lines processed by this launch
+  int32_t line_begin;                   // first line (index space of all lines)
   int32_t kind;                         // LineKind
   int32_t n;                            // transform length (real length for R2C/C2R)
   int32_t inverse;                      // 1: exponent +2 pi i jk/n
   int32_t lines_fast;                   // 1: consecutive lanes walk across lines
-  int32_t lines_per_block;
+  int32_t lines_per_block;              // 0: launcher heuristic
+  int32_t values_per_thread;            // 0: default (8); compile-time kernels exist for 4, 8, 16
   const void* tw;                       // exp(-2 pi i e / n), e in [0, n), Real precision
   double scale;
 };

@@ -108,22 +108,19 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
   }
 
   check_hip(hipGetDevice(&device_), "hipGetDevice");
-  // twiddle tables exp(-2 pi i e / n), evaluated in long double, rounded once
+  // twiddle tables exp(-2 pi i e / n), evaluated in long double and rounded
+  // once to FP64 -- for both precisions: the kernels form the powers of a
+  // butterfly in FP64 and round each once to the compute precision
   for (std::size_t k = 0; k < axes_.size(); ++k) {
     const int n = static_cast<int>(fft_extents_[k]);
     if (tw_.count(n)) continue;
-    const std::size_t es = fp64_ ? 16 : 8;
+    const std::size_t es = 16;
     std::vector<unsigned char> host(es * n);
     for (int e = 0; e < n; ++e) {
       const long double a = 6.283185307179586476925286766559005768L * e / n;
       const long double c = cosl(a), s = -sinl(a);
-      if (fp64_) {
-        double v[2] = {static_cast<double>(c), static_cast<double>(s)};
-        std::memcpy(&host[es * e], v, es);
-      } else {
-        float v[2] = {static_cast<float>(c), static_cast<float>(s)};
-        std::memcpy(&host[es * e], v, es);
-      }
+      double v[2] = {static_cast<double>(c), static_cast<double>(s)};
+      std::memcpy(&host[es * e], v, es);
     }
     void* p = nullptr;
     hipError_t err = hipMalloc(&p, std::max<std::size_t>(host.size(), 16));
@@ -295,7 +292,7 @@ DDesc Plan::scratch_desc(const Vec& ext, void* p) const {
 }
 
 LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, bool inverse,
-                         double scale) const {
+                         double scale, int slow_axis) const {
   LinePass p{};
   p.in = in;
   p.out = out;
@@ -305,6 +302,8 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   p.inverse = inverse ? 1 : 0;
   p.scale = scale;
   p.tw = twiddles(s.n);
+  p.lines_per_block = s.lines_per_block;
+  p.values_per_thread = s.values_per_thread;
   // batch axes ordered slowest -> fastest in storage order
   std::vector<int> order;
   if (in.layout_left) {
@@ -313,6 +312,11 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
     for (std::size_t k = 0; k < rank_; ++k) order.push_back(in.map[k]);
   }
   const int innermost = order.back();
+  if (slow_axis >= 0 && slow_axis != innermost) {
+    // a chunked pass enumerates the chunk axis slowest so a chunk is a contiguous range of lines
+    order.erase(std::find(order.begin(), order.end(), slow_axis));
+    order.insert(order.begin(), slow_axis);
+  }
   p.nbatch = 0;
   int64_t nl = 1;
   for (int a : order) {
@@ -325,7 +329,8 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   }
   if (nl >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many lines in one pass");
   p.nlines = static_cast<int32_t>(nl);
-  p.lines_fast = s.axis != innermost ? 1 : 0;
+  p.line_begin = 0;
+  p.lines_fast = s.lines_fast >= 0 ? s.lines_fast : (s.axis != innermost ? 1 : 0);
   return p;
 }
 
@@ -344,7 +349,41 @@ void Plan::run_fused(void* const* in, void* const* out, bool forward, hipStream_
       for (std::size_t q = 0; q < P_; ++q)
         if (in[r] != nullptr && in[r] == out[q]) alias = true;
   }
-  for (std::size_t i = 0; i < passes.size(); ++i) {
+  // Chunking: with three passes, the first two touch each plane of the third
+  // pass's axis independently, so they run plane-block by plane-block; a block
+  // small enough for the 256 MB Infinity Cache is written by pass 1 and read
+  // back by pass 2 without a round trip to HBM.
+  const int chunk = chunk_[forward ? 0 : 1];
+  std::size_t i0 = 0;
+  LinePass p0{}, p1{};
+  bool chunked = false;
+  if (chunk > 0 && passes.size() == 3 && !alias) {
+    const int cax = passes[2].axis;
+    p0 = make_pass(passes[0], src, scr, !forward, 1.0, cax);
+    p1 = make_pass(passes[1], scr, scr, !forward, 1.0, cax);
+    // the chunk axis must be the slowest batch axis of both passes
+    chunked = p0.nbatch > 0 && p1.nbatch > 0 && p0.batch_axis[0] == cax && p1.batch_axis[0] == cax;
+  }
+  if (chunked) {
+    const int cax = passes[2].axis;
+    const int extc = static_cast<int>(gout_[cax]);
+    const int32_t per0 = p0.nlines / extc, per1 = p1.nlines / extc;
+    for (int c0 = 0; c0 < extc; c0 += chunk) {
+      const int nc = std::min(chunk, extc - c0);
+      p0.line_begin = c0 * per0;
+      p0.nlines = nc * per0;
+      p1.line_begin = c0 * per1;
+      p1.nlines = nc * per1;
+      mark(stream);
+      check_hip(launch_line_pass(p0, fp64_, stream), "line pass");
+      mark(stream);
+      mark(stream);
+      check_hip(launch_line_pass(p1, fp64_, stream), "line pass");
+      mark(stream);
+    }
+    i0 = 2;
+  }
+  for (std::size_t i = i0; i < passes.size(); ++i) {
     const bool first = i == 0, last = i + 1 == passes.size();
     DDesc a = first ? src : scr;
     DDesc b = last ? (alias ? scratch_desc(forward ? gout_ : gin_, scratch_) : dst) : scr;
@@ -417,6 +456,131 @@ std::vector<float> Plan::execute_timed(void* const* in, void* const* out, int di
   for (auto e : ev) (void)hipEventDestroy(e);
   check_hip(err, "timed execute");
   return ms;
+}
+
+void Plan::set_option(const std::string& key, long value) {
+  if (key == "chunk") {
+    chunk_[0] = chunk_[1] = static_cast<int>(std::max(0L, value));
+    return;
+  }
+  if (key == "fwd.chunk" || key == "bwd.chunk") {
+    chunk_[key[0] == 'f' ? 0 : 1] = static_cast<int>(std::max(0L, value));
+    return;
+  }
+  // "<dir>.<i>.<field>"
+  const auto d1 = key.find('.'), d2 = key.rfind('.');
+  if (d1 == std::string::npos || d2 == d1) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
+  const std::string dir = key.substr(0, d1), field = key.substr(d2 + 1);
+  auto& passes = dir == "fwd" ? fwd_ : bwd_;
+  if (dir != "fwd" && dir != "bwd") throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
+  const std::size_t i = static_cast<std::size_t>(std::stoul(key.substr(d1 + 1, d2 - d1 - 1)));
+  if (i >= passes.size()) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "pass index out of range in " + key);
+  if (field == "lpb") passes[i].lines_per_block = static_cast<int>(value);
+  else if (field == "vpt") passes[i].values_per_thread = static_cast<int>(value);
+  else if (field == "lf") passes[i].lines_fast = static_cast<int>(value);
+  else throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
+}
+
+long Plan::get_option(const std::string& key) const {
+  if (key == "chunk" || key == "fwd.chunk") return chunk_[0];
+  if (key == "bwd.chunk") return chunk_[1];
+  const auto d1 = key.find('.'), d2 = key.rfind('.');
+  if (d1 == std::string::npos || d2 == d1) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
+  const std::string dir = key.substr(0, d1), field = key.substr(d2 + 1);
+  const auto& passes = dir == "fwd" ? fwd_ : bwd_;
+  const std::size_t i = static_cast<std::size_t>(std::stoul(key.substr(d1 + 1, d2 - d1 - 1)));
+  if (i >= passes.size()) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "pass index out of range in " + key);
+  if (field == "lpb") return passes[i].lines_per_block;
+  if (field == "vpt") return passes[i].values_per_thread;
+  if (field == "lf") return passes[i].lines_fast;
+  throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
+}
+
+float Plan::time_direction(void* const* in, void* const* out, bool forward, hipStream_t s, int reps) {
+  hipEvent_t e0, e1;
+  check_hip(hipEventCreate(&e0), "hipEventCreate");
+  check_hip(hipEventCreate(&e1), "hipEventCreate");
+  execute(in, out, forward ? DFFT_FORWARD : DFFT_BACKWARD, s);
+  check_hip(hipEventRecord(e0, s), "hipEventRecord");
+  for (int k = 0; k < reps; ++k) execute(in, out, forward ? DFFT_FORWARD : DFFT_BACKWARD, s);
+  check_hip(hipEventRecord(e1, s), "hipEventRecord");
+  check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+  float ms = 0.f;
+  check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
+// Autotuner: per pass, every compile-time launch shape (values per thread x
+// lines per block x lane order) is timed on the pass's own descriptors; then
+// the chunk size of the first two passes.  Shapes the launcher rejects
+// (too many threads or too much LDS) are skipped before launch.
+void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
+  if (d_.exec_mode != DFFT_EXEC_FUSED) return;
+  check_hip(hipSetDevice(device_), "hipSetDevice");
+  hipEvent_t e0, e1;
+  check_hip(hipEventCreate(&e0), "hipEventCreate");
+  check_hip(hipEventCreate(&e1), "hipEventCreate");
+  const int vpts[] = {4, 8, 16};
+  const int lpbs[] = {1, 2, 4, 8, 16, 32};
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool forward = dir == 0;
+    auto& passes = forward ? fwd_ : bwd_;
+    void* const* a_ptr = forward ? in : out;
+    void* const* b_ptr = forward ? out : in;
+    DDesc src = forward ? user_desc(gin_, in_topo_, in_tl_, a_ptr) : user_desc(gout_, out_topo_, out_tl_, a_ptr);
+    DDesc dst = forward ? user_desc(gout_, out_topo_, out_tl_, b_ptr) : user_desc(gin_, in_topo_, in_tl_, b_ptr);
+    DDesc scr = scratch_desc(gout_, scratch_);
+    for (std::size_t i = 0; i < passes.size(); ++i) {
+      if (!has_fast_line_kernel(passes[i].n)) continue;
+      DDesc a = i == 0 ? src : scr;
+      DDesc b = i + 1 == passes.size() ? dst : scr;
+      float best = 1e30f;
+      PassSpec best_spec = passes[i];
+      for (int vpt : vpts) {
+        for (int lpb : lpbs) {
+          for (int lf = 0; lf < 2; ++lf) {
+            PassSpec s = passes[i];
+            s.values_per_thread = vpt;
+            s.lines_per_block = lpb;
+            s.lines_fast = lf;
+            LinePass p = make_pass(s, a, b, !forward, 1.0);
+            if (launch_line_pass(p, fp64_, stream) != hipSuccess) {
+              (void)hipGetLastError();
+              continue;
+            }
+            check_hip(hipEventRecord(e0, stream), "hipEventRecord");
+            for (int k = 0; k < 3; ++k) check_hip(launch_line_pass(p, fp64_, stream), "line pass");
+            check_hip(hipEventRecord(e1, stream), "hipEventRecord");
+            check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            if (ms < best) {
+              best = ms;
+              best_spec = s;
+            }
+          }
+        }
+      }
+      passes[i] = best_spec;
+    }
+    if (passes.size() == 3) {
+      int best_chunk = 0;
+      float best = 1e30f;
+      for (int c : {0, 4, 8, 16, 32, 64}) {
+        chunk_[dir] = c;
+        const float ms = time_direction(a_ptr, b_ptr, forward, stream, 3);
+        if (ms < best) {
+          best = ms;
+          best_chunk = c;
+        }
+      }
+      chunk_[dir] = best_chunk;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
 }
 
 int Plan::num_passes(int direction) const {

@@ -19,6 +19,10 @@ struct PassSpec {
   int axis;
   int kind;  // LineKind
   int n;     // transform length
+  // launch shape (0 / -1 = launcher heuristic); set by options or the autotuner
+  int lines_per_block = 0;
+  int values_per_thread = 0;
+  int lines_fast = -1;
 };
 
 class Plan {
@@ -32,6 +36,14 @@ class Plan {
   // execute with a HIP event pair around every launch; durations in ms
   std::vector<float> execute_timed(void* const* in, void* const* out, int direction, hipStream_t stream);
   int num_passes(int direction) const;
+  // tuning knobs: "chunk" (planes per chunk of the first two passes, 0 = off),
+  // "fwd.<i>.lpb|vpt|lf", "bwd.<i>.lpb|vpt|lf" (lines per block, values per
+  // thread, lanes-across-lines) of pass i
+  void set_option(const std::string& key, long value);
+  long get_option(const std::string& key) const;
+  // measure candidate launch shapes and chunk sizes on the caller's arrays
+  // (their contents are overwritten) and keep the fastest
+  void autotune(void* const* in, void* const* out, hipStream_t stream);
 
   const std::string& name() const { return name_; }
   std::size_t nranks() const { return P_; }
@@ -45,7 +57,9 @@ class Plan {
  private:
   DDesc user_desc(const Vec& ext, const Vec& topo, Layout tl, void* const* ptrs) const;
   DDesc scratch_desc(const Vec& ext, void* p) const;
-  LinePass make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, bool inverse, double scale) const;
+  LinePass make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, bool inverse, double scale,
+                     int slow_axis = -1) const;
+  float time_direction(void* const* in, void* const* out, bool forward, hipStream_t s, int reps);
   void plan_fused();
   void run_fused(void* const* in, void* const* out, bool forward, hipStream_t s);
   void run_blockwalk(void* const* in, void* const* out, bool forward, hipStream_t s);
@@ -69,6 +83,7 @@ class Plan {
   std::size_t workspace_bytes_ = 0;
   std::map<int, void*> tw_;
   std::vector<PassSpec> fwd_, bwd_;
+  int chunk_[2] = {0, 0};  // planes per chunk, forward / backward (0 = no chunking)
   // timing hooks (set only by execute_timed)
   std::vector<hipEvent_t>* timing_ = nullptr;
   void mark(hipStream_t s);

@@ -146,7 +146,7 @@ EXPORTED = [
     "dfft_last_error", "dfft_version", "dfft_plan_create", "dfft_plan_destroy", "dfft_plan_name",
     "dfft_plan_local_extents", "dfft_plan_num_blocks", "dfft_plan_block_info",
     "dfft_plan_max_buffer_size", "dfft_plan_workspace_bytes", "dfft_execute", "dfft_plan_num_passes",
-    "dfft_execute_timed", "dfft_pack",
+    "dfft_execute_timed", "dfft_plan_set_option", "dfft_plan_get_option", "dfft_plan_autotune", "dfft_pack",
     "dfft_unpack", "dfft_all2all", "dfft_exchange", "dfft_get_all_slab_topologies",
     "dfft_get_all_pencil_topologies", "dfft_block_analyses",
 ]
@@ -183,6 +183,9 @@ def load_library(path: os.PathLike | None = None):
         "dfft_plan_num_passes": ([ctypes.c_void_p, ctypes.c_int, ip], ctypes.c_int),
         "dfft_execute_timed": ([ctypes.c_void_p, vpp, vpp, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip], ctypes.c_int),
+        "dfft_plan_set_option": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long], ctypes.c_int),
+        "dfft_plan_get_option": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "dfft_plan_autotune": ([ctypes.c_void_p, vpp, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_pack": ([ctypes.c_void_p, szp, szp, ctypes.c_int, ctypes.c_void_p, szp, ctypes.c_int,
                        ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
         "dfft_unpack": ([ctypes.c_void_p, szp, ctypes.c_int, ctypes.c_void_p, szp, szp, ctypes.c_int,
@@ -427,6 +430,28 @@ class Plan:
         _check(self._lib.dfft_execute_timed(self._plan, _ptrs(ins), _ptrs(outs), int(direction),
                                             _stream_handle(stream), buf, cap, ctypes.byref(n)))
         return [float(buf[i]) for i in range(min(n.value, cap))]
+
+    def set_option(self, key, value):
+        _check(self._lib.dfft_plan_set_option(self._plan, key.encode(), int(value)))
+
+    def get_option(self, key):
+        v = ctypes.c_long()
+        _check(self._lib.dfft_plan_get_option(self._plan, key.encode(), ctypes.byref(v)))
+        return int(v.value)
+
+    def options(self):
+        """Current launch shapes of every pass and the chunk sizes."""
+        out = {"fwd.chunk": self.get_option("fwd.chunk"), "bwd.chunk": self.get_option("bwd.chunk")}
+        for d, dirn in (("fwd", Direction.forward), ("bwd", Direction.backward)):
+            for i in range(max(0, self.num_passes(dirn))):
+                for f in ("lpb", "vpt", "lf"):
+                    out[f"{d}.{i}.{f}"] = self.get_option(f"{d}.{i}.{f}")
+        return out
+
+    def autotune(self, in_views, out_views, stream=None):
+        """Measure launch shapes on these arrays (their contents are overwritten)."""
+        _check(self._lib.dfft_plan_autotune(self._plan, _ptrs(in_views), _ptrs(out_views),
+                                            _stream_handle(stream)))
 
     def destroy(self):
         if getattr(self, "_plan", None) and self._plan.value:

@@ -133,6 +133,18 @@ int dfft_plan_num_passes(const dfft_plan* plan, int direction, int* npasses);
 int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int direction, void* stream,
                        float* ms_per_pass, int capacity, int* npasses);
 
+/* Launch-shape knobs of a fused plan (no counterpart in the reference, whose
+ * vendor FFT plans choose their own):  "chunk" / "fwd.chunk" / "bwd.chunk" =
+ * planes per chunk of the first two passes (0 = off); "fwd.<i>.lpb",
+ * "fwd.<i>.vpt", "fwd.<i>.lf" (and "bwd.…") = lines per block, values per
+ * thread (4, 8, 16) and lane order of pass i. */
+int dfft_plan_set_option(dfft_plan* plan, const char* key, long value);
+int dfft_plan_get_option(const dfft_plan* plan, const char* key, long* value);
+/* Autotuner: times every launch shape of every pass and the chunk sizes on the
+ * given arrays and keeps the fastest.  Like FFTW_MEASURE planning it
+ * overwrites the contents of both `in` and `out`. */
+int dfft_plan_autotune(dfft_plan* plan, void* const* in, void* const* out, void* stream);
+
 /* ---- exchange layer (standalone, one GPU) ------------------------------ */
 /* pack() (PackUnpack.hpp:439-460) of one rank's local array `src` (storage
  * extents src_extents, mapped by src_map, LayoutRight) into the buffer

@@ -39,7 +39,7 @@ def gather_rel_l2(gpu_locals, ref_locals):
 
 
 def run_gpu_plan(dev, x, gin, axes, tin, tout, real, fp64=True, norm="backward", exec_mode=0,
-                 tin_layout=None, tout_layout=None):
+                 tin_layout=None, tout_layout=None, configure=None):
     """Forward + backward through the C ABI; returns (outs, backs, plan) as numpy."""
     import torch
     import dfft
@@ -59,6 +59,10 @@ def run_gpu_plan(dev, x, gin, axes, tin, tout, real, fp64=True, norm="backward",
         outs.append(torch.full(e, float("nan"), dtype=cdt, device=dev))
     plan = dfft.Plan(ins, outs, axes, tin_t, tout_t, norm=norm, exec_mode=exec_mode,
                      global_in_extents=gin, global_out_extents=gout)
+    if configure is not None:
+        configure(plan, ins, outs)
+        ins = [torch.from_numpy(np.ascontiguousarray(a)).to(dev, rdt if real else cdt)
+               for a in PE.scatter(x, gin, tin, lin)]
     dfft.execute(plan, ins, outs, dfft.Direction.forward)
     backs = [torch.full(list(a.shape), float("nan"), dtype=a.dtype, device=dev) for a in ins]
     dfft.execute(plan, outs, backs, dfft.Direction.backward)

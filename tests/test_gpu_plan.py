@@ -137,6 +137,29 @@ def test_layout_left_topologies(dev):
     _check(dev, [8, 7, 6], [0, 1, 2], [2, 3, 1], [1, 2, 3], True, tin_layout="left", tout_layout="right")
 
 
+def test_chunked_and_tuned_launch_shapes(dev):
+    """Launch-shape options, chunked first passes and the autotuner change the
+    schedule, never the result."""
+    def chunk(c):
+        return lambda plan, ins, outs: plan.set_option("chunk", c)
+
+    def shapes(plan, ins, outs):
+        for d in ("fwd", "bwd"):
+            for i in range(3):
+                plan.set_option(f"{d}.{i}.vpt", 16)
+                plan.set_option(f"{d}.{i}.lpb", 2)
+                plan.set_option(f"{d}.{i}.lf", 1)
+
+    def tune(plan, ins, outs):
+        plan.autotune(ins, outs)
+        assert plan.get_option("fwd.chunk") in (0, 4, 8, 16, 32, 64)
+
+    for cfg in (chunk(4), chunk(7), shapes, tune):
+        _check(dev, [32, 24, 64], [0, 1, 2], [2, 1, 1], [1, 2, 1], False, seed=21, configure=cfg)
+        _check(dev, [32, 32, 64], [0, 1, 2], [2, 2, 1], [1, 2, 2], True, seed=22, configure=cfg)
+        _check(dev, [16, 64, 32], [2, 0, 1], [1, 1, 2], [2, 1, 1], True, fp64=False, seed=23, configure=cfg)
+
+
 def test_nonsmooth_lengths_generic_kernel(dev):
     """Prime and mixed lengths go through the generic kernel."""
     for gin in ([11, 13, 17], [9, 25, 49], [96, 10, 160]):
