@@ -7,11 +7,16 @@
 
 #include "dfft.h"
 #include "fft_kernels.hpp"
+#include "navier_stokes.hpp"
 #include "plan.hpp"
 #include "planner.hpp"
 
 struct dfft_plan {
   dfft::Plan* impl;
+};
+
+struct dfft_ns {
+  dfft::NavierStokes* impl;
 };
 
 namespace {
@@ -343,6 +348,112 @@ int dfft_exchange(void* const* in, void* const* out, int rank, const size_t* glo
         if (in[r] == out[q] && in[r] != nullptr)
           throw dfft::Error(DFFT_ERROR_ALIAS, "input and output local arrays must not alias");
     hip_ok(dfft::launch_remap(ra, elem_size, static_cast<hipStream_t>(stream)), "exchange");
+  });
+}
+
+int dfft_ns_create(const dfft_ns_desc* d, dfft_ns** ns) {
+  return guarded([&] {
+    need(d && ns, "null pointer");
+    *ns = nullptr;
+    auto* h = new dfft_ns{nullptr};
+    try {
+      h->impl = new dfft::NavierStokes(d->n, d->px, d->py, d->lx, d->nu, d->dt, d->form);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *ns = h;
+  });
+}
+
+int dfft_ns_destroy(dfft_ns* ns) {
+  return guarded([&] {
+    if (!ns) return;
+    delete ns->impl;
+    delete ns;
+  });
+}
+
+int dfft_ns_local_extents(const dfft_ns* ns, int vrank, int which, size_t* extents, size_t* starts) {
+  return guarded([&] {
+    need(ns && extents, "null pointer");
+    need(vrank >= 0 && vrank < ns->impl->nranks(), "rank out of range");
+    const dfft::NsBlock& b = which == 0 ? ns->impl->physical_block(vrank) : ns->impl->spectral_block(vrank);
+    for (int a = 0; a < 3; ++a) {
+      extents[a] = static_cast<size_t>(b.ext[a]);
+      if (starts) starts[a] = static_cast<size_t>(b.start[a]);
+    }
+  });
+}
+
+int dfft_ns_init_taylor_green(dfft_ns* ns, double v0, void* stream) {
+  return guarded([&] {
+    need(ns, "null pointer");
+    ns->impl->init_taylor_green(v0, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_set_physical(dfft_ns* ns, void* const* u3, void* stream) {
+  return guarded([&] {
+    need(ns && u3, "null pointer");
+    ns->impl->set_physical(u3, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_set_state(dfft_ns* ns, void* const* y3, void* stream) {
+  return guarded([&] {
+    need(ns && y3, "null pointer");
+    ns->impl->set_state(y3, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_get_state(dfft_ns* ns, void* const* y3, void* stream) {
+  return guarded([&] {
+    need(ns && y3, "null pointer");
+    ns->impl->get_state(y3, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_step(dfft_ns* ns, int nsteps, int use_graph, void* stream) {
+  return guarded([&] {
+    need(ns, "null pointer");
+    ns->impl->step(nsteps, use_graph != 0, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_energy(dfft_ns* ns, double* energy, void* stream) {
+  return guarded([&] {
+    need(ns && energy, "null pointer");
+    *energy = ns->impl->energy(static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_rhs(dfft_ns* ns, void* const* y3, void* const* out3, void* stream) {
+  return guarded([&] {
+    need(ns && y3 && out3, "null pointer");
+    ns->impl->apply_rhs(y3, out3, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_rhs_tangent(dfft_ns* ns, void* const* y3, void* const* d3, void* const* out3, void* stream) {
+  return guarded([&] {
+    need(ns && y3 && d3 && out3, "null pointer");
+    ns->impl->apply_rhs_tangent(y3, d3, out3, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_rhs_adjoint(dfft_ns* ns, void* const* y3, void* const* l3, void* const* out3, void* stream) {
+  return guarded([&] {
+    need(ns && y3 && l3 && out3, "null pointer");
+    ns->impl->apply_rhs_adjoint(y3, l3, out3, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_gradient(dfft_ns* ns, int nsteps, int checkpoint_every, void* const* u0, void* const* grad, double* J,
+                     void* stream) {
+  return guarded([&] {
+    need(ns && u0 && grad && J, "null pointer");
+    *J = ns->impl->gradient(nsteps, checkpoint_every, u0, grad, static_cast<hipStream_t>(stream));
   });
 }
 

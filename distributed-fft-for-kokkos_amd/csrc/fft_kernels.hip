@@ -494,10 +494,14 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
   const size_t smem = static_cast<size_t>(B) * Cfg::LDSN * sizeof(C);
   if (B * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
   p.lines_per_block = B;
-  if (smem > 64 * 1024) {
+  // raise the dynamic-LDS limit once per kernel instance (never inside a
+  // stream capture that follows an eager launch of the same shape)
+  static size_t lds_limit = 64 * 1024;
+  if (smem > lds_limit) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
     if (e != hipSuccess) return e;
+    lds_limit = smem;
   }
   const unsigned grid = static_cast<unsigned>((p.nlines + B - 1) / B);
   if (grid == 0) return hipSuccess;
@@ -535,10 +539,12 @@ hipError_t launch_generic(LinePass p, hipStream_t s) {
   while (B > 1 && static_cast<size_t>(2) * B * p.n * sizeof(C) > 64 * 1024) B /= 2;
   const size_t smem = static_cast<size_t>(2) * B * p.n * sizeof(C);
   if (smem > 160 * 1024) return hipErrorInvalidValue;
-  if (smem > 64 * 1024) {
+  static size_t lds_limit = 64 * 1024;
+  if (smem > lds_limit) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_line_generic<Real>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
     if (e != hipSuccess) return e;
+    lds_limit = smem;
   }
   p.lines_per_block = B;
   const unsigned grid = static_cast<unsigned>((p.nlines + B - 1) / B);

@@ -1,0 +1,74 @@
+// Pseudo-spectral incompressible Navier-Stokes solver on P virtual ranks
+// (the caller of the distributed plan in examples/navier-stokes-MPI and
+// examples/navier-stokes-MPI-batched).
+//
+// Every field set is held as in the batched example: a leading field axis,
+// physical blocks in the (1, px, py, 1) topology, spectral blocks in
+// (1, 1, px, py).  Each RK stage runs two batched plan executions: the
+// backward transform of all fields it needs and the forward transform of the
+// nonlinear terms -- each line pass is one kernel launch over every field.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "ns_kernels.hpp"
+#include "plan.hpp"
+
+namespace dfft {
+
+class NavierStokes {
+ public:
+  // form 0: divergence form, 3 C2R + 6 R2C per stage (u_i u_j products);
+  // form 1: the reference's advective form, 12 C2R + 3 R2C per stage.
+  NavierStokes(int n, int px, int py, double lx, double nu, double dt, int form);
+  ~NavierStokes();
+  NavierStokes(const NavierStokes&) = delete;
+  NavierStokes& operator=(const NavierStokes&) = delete;
+
+  void init_taylor_green(double v0, hipStream_t s);
+  void step(int nsteps, bool use_graph, hipStream_t s);
+  double energy(hipStream_t s);
+  void get_state(void* const* dst, hipStream_t s) const;
+  void set_state(void* const* src, hipStream_t s);
+  void set_physical(void* const* u3, hipStream_t s);  // physical velocity -> dealiased, projected spectrum
+  const NsBlock& spectral_block(int r) const { return sblk_.at(r); }
+  const NsBlock& physical_block(int r) const { return pblk_.at(r); }
+  int nranks() const { return P_; }
+
+  // reverse mode (ns_adjoint.cpp)
+  double gradient(int nsteps, int checkpoint_every, void* const* u0, void* const* grad, hipStream_t s);
+  void apply_rhs(void* const* y3, void* const* out3, hipStream_t s);          // R(y)
+  void apply_rhs_tangent(void* const* y3, void* const* d3, void* const* out3, hipStream_t s);  // R'(y) d
+  void apply_rhs_adjoint(void* const* y3, void* const* l3, void* const* out3, hipStream_t s);  // R'(y)^* l
+
+ private:
+  void rk_stage(int stage, hipStream_t s);
+  void one_step(hipStream_t s);
+  int64_t spec_pts(int r) const;
+  int64_t phys_pts(int r) const;
+  std::vector<void*> alloc_set(bool spectral, int nf);
+  void ensure_aux();
+
+  int n_, px_, py_, P_, form_;
+  NsGrid g_{};
+  double lx_;
+  std::unique_ptr<Plan> plan_b_, plan_f_;  // backward (nf_b fields), forward (nf_f fields)
+  std::unique_ptr<Plan> plan_b3_, plan_f3_;  // 3-field plans (init, adjoint)
+  std::unique_ptr<Plan> plan_b6_;            // 6-field backward (adjoint)
+  int nf_b_, nf_f_;
+  std::vector<NsBlock> sblk_, pblk_;  // per rank (3-field blocks; nf is set per use)
+  // device buffers, one pointer per rank
+  std::vector<void*> y_, yc_, k1_, k2_, k3_, nl_, bin_, pb_, pf_;
+  std::vector<void*> aux_s3a_, aux_s3b_, aux_p3_;  // tangent/adjoint work fields (allocated on first use)
+  std::vector<void*> all_;
+  double* partials_ = nullptr;
+  double* esum_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipStream_t graph_stream_ = nullptr;
+};
+
+}  // namespace dfft

@@ -421,7 +421,9 @@ void Plan::execute(void* const* in, void* const* out, int direction, hipStream_t
     if (total_size(ie) > 0 && in[r] == nullptr) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null input pointer");
     if (total_size(oe) > 0 && out[r] == nullptr) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null output pointer");
   }
-  check_hip(hipSetDevice(device_), "hipSetDevice");
+  int cur = -1;
+  check_hip(hipGetDevice(&cur), "hipGetDevice");
+  if (cur != device_) check_hip(hipSetDevice(device_), "hipSetDevice");
   if (total_size(gin_) == 0) return;
   if (d_.exec_mode == DFFT_EXEC_BLOCKWALK) run_blockwalk(in, out, forward, stream);
   else run_fused(in, out, forward, stream);

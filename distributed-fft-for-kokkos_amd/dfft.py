@@ -142,13 +142,28 @@ class BlockInfo(ctypes.Structure):
         }
 
 
+class NsDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("px", ctypes.c_int),
+        ("py", ctypes.c_int),
+        ("lx", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("form", ctypes.c_int),
+    ]
+
+
 EXPORTED = [
     "dfft_last_error", "dfft_version", "dfft_plan_create", "dfft_plan_destroy", "dfft_plan_name",
     "dfft_plan_local_extents", "dfft_plan_num_blocks", "dfft_plan_block_info",
     "dfft_plan_max_buffer_size", "dfft_plan_workspace_bytes", "dfft_execute", "dfft_plan_num_passes",
     "dfft_execute_timed", "dfft_plan_set_option", "dfft_plan_get_option", "dfft_plan_autotune", "dfft_pack",
     "dfft_unpack", "dfft_all2all", "dfft_exchange", "dfft_get_all_slab_topologies",
-    "dfft_get_all_pencil_topologies", "dfft_block_analyses",
+    "dfft_get_all_pencil_topologies", "dfft_block_analyses", "dfft_ns_create", "dfft_ns_destroy",
+    "dfft_ns_local_extents", "dfft_ns_init_taylor_green", "dfft_ns_set_physical", "dfft_ns_set_state",
+    "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_energy", "dfft_ns_rhs", "dfft_ns_rhs_tangent",
+    "dfft_ns_rhs_adjoint", "dfft_ns_gradient",
 ]
 
 _lib = None
@@ -198,6 +213,20 @@ def load_library(path: os.PathLike | None = None):
                                             ip, szp, szp, szp, ip], ctypes.c_int),
         "dfft_block_analyses": ([ctypes.POINTER(_Desc), ctypes.c_int, ctypes.POINTER(BlockInfo), ip, szp],
                                 ctypes.c_int),
+        "dfft_ns_create": ([ctypes.POINTER(NsDesc), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "dfft_ns_destroy": ([ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_local_extents": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, szp, szp], ctypes.c_int),
+        "dfft_ns_init_taylor_green": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_set_physical": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_set_state": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_get_state": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_step": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_energy": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_rhs": ([ctypes.c_void_p, vpp, vpp, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_rhs_tangent": ([ctypes.c_void_p, vpp, vpp, vpp, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_rhs_adjoint": ([ctypes.c_void_p, vpp, vpp, vpp, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_gradient": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, vpp, vpp,
+                              ctypes.POINTER(ctypes.c_double), ctypes.c_void_p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

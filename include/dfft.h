@@ -166,6 +166,46 @@ int dfft_exchange(void* const* in, void* const* out, int rank, const size_t* glo
                   const size_t* out_topology, int out_topology_layout, const size_t* out_map, int elem_size,
                   void* stream);
 
+/* ---- pseudo-spectral Navier-Stokes (examples/navier-stokes-MPI) -------- */
+/* The example's solver (main.cpp:31-986, utils/math_utils.hpp:47-141) on P =
+ * px*py virtual ranks: velocity spectra in the (1,1,px,py) topology of the
+ * batched example (navier-stokes-MPI-batched/main.cpp:102-107), fields as a
+ * leading axis of 3.  form 0 evaluates u.grad(u) as div(u u) (3 C2R + 6 R2C
+ * per RK stage); form 1 is the example's advective form (12 C2R + 3 R2C). */
+typedef struct dfft_ns dfft_ns;
+typedef struct dfft_ns_desc {
+  int n;      /* grid points per direction */
+  int px, py; /* virtual process grid */
+  double lx;  /* domain [0, lx*pi)^3 */
+  double nu;  /* 1/Re */
+  double dt;
+  int form;   /* 0 divergence form, 1 advective form */
+} dfft_ns_desc;
+int dfft_ns_create(const dfft_ns_desc* desc, dfft_ns** ns);
+int dfft_ns_destroy(dfft_ns* ns);
+/* which = 0: physical block (3, ext...) of rank r; 1: spectral block */
+int dfft_ns_local_extents(const dfft_ns* ns, int vrank, int which, size_t* extents, size_t* starts);
+/* Taylor-Green initial condition (main.cpp:349-353, 398-417) */
+int dfft_ns_init_taylor_green(dfft_ns* ns, double v0, void* stream);
+/* physical velocity (P blocks of (3, ...) float64) -> dealiased, projected state */
+int dfft_ns_set_physical(dfft_ns* ns, void* const* u3, void* stream);
+int dfft_ns_set_state(dfft_ns* ns, void* const* y3, void* stream);
+int dfft_ns_get_state(dfft_ns* ns, void* const* y3, void* stream);
+/* RK4 steps (NavierStokes::solve, main.cpp:544-561); use_graph replays one
+ * captured step as a hipGraph after an eager first step */
+int dfft_ns_step(dfft_ns* ns, int nsteps, int use_graph, void* stream);
+/* kinetic energy, FP64 reduction (diag_energy, main.cpp:855-910) */
+int dfft_ns_energy(dfft_ns* ns, double* energy, void* stream);
+/* right-hand side R(y), its tangent R'(y)d and adjoint R'(y)^*l (form 0);
+ * no counterpart in the reference -- for gradient-based callers */
+int dfft_ns_rhs(dfft_ns* ns, void* const* y3, void* const* out3, void* stream);
+int dfft_ns_rhs_tangent(dfft_ns* ns, void* const* y3, void* const* d3, void* const* out3, void* stream);
+int dfft_ns_rhs_adjoint(dfft_ns* ns, void* const* y3, void* const* l3, void* const* out3, void* stream);
+/* J = kinetic energy after nsteps RK4 steps from the physical velocity u0,
+ * and dJ/du0, by checkpointed adjoint recomputation (form 0) */
+int dfft_ns_gradient(dfft_ns* ns, int nsteps, int checkpoint_every, void* const* u0, void* const* grad, double* J,
+                     void* stream);
+
 /* ---- planner (host only, no device needed) ----------------------------- */
 /* get_all_slab_topologies (Topologies.hpp:420-618); topologies written as
  * ntopo x rank entries; *ntopo holds the capacity on entry. */

@@ -1,0 +1,207 @@
+"""GPU Navier-Stokes solver (examples/navier-stokes-MPI) against the oracle,
+and its tangent / adjoint / checkpointed gradient."""
+
+import numpy as np
+import pytest
+
+from oracle import navier_stokes as NS
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(n, px, py, form, nu=1.0 / 1600.0, dt=1e-3):
+    import dfft_ns
+    return dfft_ns.NavierStokes(n, px, py, lx=2.0, nu=nu, dt=dt, form=form)
+
+
+def _scatter(s, arrs, which, dev):
+    """Global (3, ...) numpy field set -> per-rank device blocks."""
+    import torch
+    out = []
+    for r in range(s.nranks):
+        e, st = s.local_extents(r, which)
+        sl = tuple(slice(a, a + b) for a, b in zip(st, e))
+        blk = np.stack([a[sl] for a in arrs])
+        dt = torch.float64 if which == 0 else torch.complex128
+        out.append(torch.from_numpy(np.ascontiguousarray(blk)).to(dev, dt))
+    return out
+
+
+def _gather(s, blocks, which):
+    n = s.n
+    shape = (n, n, n) if which == 0 else (n, n, n // 2 + 1)
+    out = np.zeros((3,) + shape, dtype=np.float64 if which == 0 else np.complex128)
+    for r, b in enumerate(blocks):
+        e, st = s.local_extents(r, which)
+        sl = (slice(None),) + tuple(slice(a, a + b) for a, b in zip(st, e))
+        out[sl] = b.cpu().numpy()
+    return out
+
+
+def _hdot(n, a, b):
+    """<a, b>_h = sum h Re(conj a b), h = 1 on the kz = 0, n/2 planes, 2 elsewhere."""
+    h = np.full(a.shape[-1], 2.0)
+    h[0] = 1.0
+    if n % 2 == 0:
+        h[n // 2] = 1.0
+    return float((h * (np.conj(a) * b).real).sum())
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+@pytest.mark.parametrize("n,px,py,form", [
+    (16, 1, 1, 0), (16, 2, 2, 0), (16, 1, 3, 0), (32, 2, 1, 0),
+    (16, 1, 1, 1), (24, 2, 2, 1), (16, 3, 2, 1),
+])
+def test_taylor_green_steps_match_oracle(dev, n, px, py, form):
+    nu, dt, nsteps = 0.01, 1e-2, 3
+    s = _solver(n, px, py, form, nu, dt)
+    s.init_taylor_green(1.0)
+    g = NS.Grid(n)
+    y = list(NS.initial_spectrum(g, *NS.taylor_green(g)))
+    assert abs(s.energy() - np.pi ** 3) < 1e-12 * np.pi ** 3
+    assert _rel(_gather(s, s.get_state(), 1), np.stack(y)) < 1e-12
+    s.step(nsteps)
+    for _ in range(nsteps):
+        y = NS.rk4_step(g, nu, dt, y)
+    got = _gather(s, s.get_state(), 1)
+    assert _rel(got, np.stack(y)) < 1e-12
+    assert abs(s.energy() - NS.kinetic_energy(g, *y)) < 1e-12 * np.pi ** 3
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_random_field_steps_match_oracle(dev, form):
+    n, nu, dt = 16, 0.02, 5e-3
+    rng = np.random.default_rng(11)
+    u = [rng.standard_normal((n, n, n)) for _ in range(3)]
+    s = _solver(n, 2, 2, form, nu, dt)
+    s.set_physical(_scatter(s, u, 0, dev))
+    g = NS.Grid(n)
+    y = list(NS.initial_spectrum(g, *u))
+    s.step(2)
+    for _ in range(2):
+        y = NS.rk4_step(g, nu, dt, y)
+    assert _rel(_gather(s, s.get_state(), 1), np.stack(y)) < 1e-12
+
+
+def test_graph_replay_matches_eager_bitwise(dev):
+    a = _solver(32, 2, 2, 0)
+    b = _solver(32, 2, 2, 0)
+    a.init_taylor_green(1.0)
+    b.init_taylor_green(1.0)
+    a.step(5, use_graph=False)
+    b.step(2, use_graph=True)   # eager first step, then a captured step replayed
+    b.step(3, use_graph=True)   # replays only
+    ya, yb = a.get_state(), b.get_state()
+    for ta, tb in zip(ya, yb):
+        assert bool((ta == tb).all())
+    assert a.energy() == b.energy()
+
+
+def test_rhs_of_taylor_green_matches_closed_form(dev):
+    n, nu = 32, 0.01
+    s = _solver(n, 2, 2, 0, nu)
+    s.init_taylor_green(1.0)
+    d = _gather(s, s.rhs(s.get_state()), 1)
+    g = NS.Grid(n)
+    X, Y, Z = np.meshgrid(g.x, g.x, g.x, indexing="ij")
+    u, v, _ = NS.taylor_green(g)
+    want = [-np.sin(2 * X) * np.cos(2 * Z) / 8.0 - 3 * nu * u,
+            -np.sin(2 * Y) * np.cos(2 * Z) / 8.0 - 3 * nu * v,
+            (np.cos(2 * X) + np.cos(2 * Y)) * np.sin(2 * Z) / 8.0]
+    assert _rel(d, np.stack([NS.forward(c) for c in want])) < 1e-12
+
+
+def _random_states(s, dev, seeds):
+    out = []
+    for sd in seeds:
+        rng = np.random.default_rng(sd)
+        u = [rng.standard_normal((s.n,) * 3) for _ in range(3)]
+        s.set_physical(_scatter(s, u, 0, dev))
+        out.append(s.get_state())
+    return out
+
+
+def test_tangent_is_exact_central_difference(dev):
+    """R is quadratic, so (R(y + e d) - R(y - e d)) / 2e equals R'(y) d up to rounding."""
+    import torch
+    s = _solver(16, 2, 2, 0, nu=0.01)
+    y, d = _random_states(s, dev, [1, 2])
+    eps = 1e-3
+    rp = s.rhs([a + eps * b for a, b in zip(y, d)])
+    rm = s.rhs([a - eps * b for a, b in zip(y, d)])
+    fd = [(p - m) / (2 * eps) for p, m in zip(rp, rm)]
+    tl = s.rhs_tangent(y, d)
+    num = sum(float(torch.linalg.vector_norm(a - b)) ** 2 for a, b in zip(tl, fd))
+    den = sum(float(torch.linalg.vector_norm(b)) ** 2 for b in fd)
+    assert (num / den) ** 0.5 < 1e-9
+
+
+@pytest.mark.parametrize("n,px,py", [(16, 1, 1), (16, 2, 2), (24, 2, 3)])
+def test_adjoint_dot_product(dev, n, px, py):
+    """<R'(y) d, l>_h == <d, R'(y)^* l>_h."""
+    s = _solver(n, px, py, 0, nu=0.01)
+    y, d, lam = _random_states(s, dev, [3, 4, 5])
+    lhs = _hdot(n, _gather(s, s.rhs_tangent(y, d), 1), _gather(s, lam, 1))
+    rhs = _hdot(n, _gather(s, d, 1), _gather(s, s.rhs_adjoint(y, lam), 1))
+    assert abs(lhs - rhs) < 1e-12 * max(abs(lhs), abs(rhs))
+
+
+def _tg_noise(n, seed, sigma=1e-2):
+    g = NS.Grid(n)
+    rng = np.random.default_rng(seed)
+    return g, [a + rng.normal(0.0, sigma, a.shape) for a in NS.taylor_green(g)]
+
+
+@pytest.mark.parametrize("nsteps,ck,px,py", [(3, 1, 1, 1), (3, 2, 2, 2), (4, 5, 1, 2)])
+def test_gradient_matches_oracle_directional_derivative(dev, nsteps, ck, px, py):
+    n, nu, dt = 16, 0.01, 1e-2
+    g, u0 = _tg_noise(n, 5)
+    rng = np.random.default_rng(6)
+    delta = [rng.standard_normal((n, n, n)) for _ in range(3)]
+    s = _solver(n, px, py, 0, nu, dt)
+    j, grad = s.gradient(nsteps, _scatter(s, u0, 0, dev), checkpoint_every=ck)
+    j_ref, dj_ref = NS.final_energy_directional(g, nu, dt, nsteps, u0, delta)
+    assert abs(j - j_ref) < 1e-12 * abs(j_ref)
+    dj = float((_gather(s, grad, 0) * np.stack(delta)).sum())
+    assert abs(dj - dj_ref) < 1e-10 * abs(dj_ref)
+
+
+def test_gradient_128_cubed_20_steps_against_central_differences(dev):
+    """Taylor-Green at 128^3, Re = 1600, dt = 1e-3, N(0, 1e-2) noise (seed 5),
+    20 RK4 steps, checkpoint every 5: dJ/du0 . delta vs central differences, eps = 1e-6."""
+    import torch
+    n, nsteps = 128, 20
+    _, u0 = _tg_noise(n, 5)
+    s = _solver(n, 1, 1, 0)
+    u0d = _scatter(s, u0, 0, dev)
+    j, grad = s.gradient(nsteps, u0d, checkpoint_every=5)
+    rng = np.random.default_rng(7)
+    directions = {"u0": u0d, "noise": _scatter(s, [rng.standard_normal((n, n, n)) for _ in range(3)], 0, dev)}
+    eps = 1e-6
+    for name, dd in directions.items():
+        jp = s.final_energy(nsteps, [a + eps * b for a, b in zip(u0d, dd)])
+        jm = s.final_energy(nsteps, [a - eps * b for a, b in zip(u0d, dd)])
+        fd = (jp - jm) / (2 * eps)
+        ad = sum(float((g_ * d_).sum()) for g_, d_ in zip(grad, dd))
+        print(f"direction {name}: adjoint {ad:.15e} fd {fd:.15e} rel {abs(ad - fd) / abs(fd):.2e}")
+        assert abs(ad - fd) < 1e-6 * abs(fd)
+    assert abs(j - s.final_energy(nsteps, u0d)) < 1e-13 * abs(j)
+    del torch
+
+
+def test_autograd_harness(dev):
+    import torch
+    import dfft_ns
+    n, nsteps = 16, 3
+    _, u0 = _tg_noise(n, 5)
+    s = _solver(n, 2, 1, 0, 0.01, 1e-2)
+    u0d = [t.requires_grad_(True) for t in _scatter(s, u0, 0, dev)]
+    e = dfft_ns.final_energy(s, nsteps, u0d, checkpoint_every=2)
+    gr = torch.autograd.grad(3.0 * e, u0d)
+    j, grad = s.gradient(nsteps, [t.detach() for t in u0d], checkpoint_every=2)
+    assert abs(float(e) - j) == 0.0
+    for a, b in zip(gr, grad):
+        assert torch.allclose(a, 3.0 * b, rtol=0, atol=0)
