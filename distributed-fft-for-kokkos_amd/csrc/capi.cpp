@@ -166,6 +166,13 @@ int dfft_execute(dfft_plan* plan, void* const* in, void* const* out, int directi
   });
 }
 
+int dfft_execute_adjoint(dfft_plan* plan, void* const* in, void* const* out, int direction, void* stream) {
+  return guarded([&] {
+    need(plan, "null plan");
+    plan->impl->execute_adjoint(in, out, direction, static_cast<hipStream_t>(stream));
+  });
+}
+
 int dfft_plan_num_passes(const dfft_plan* plan, int direction, int* npasses) {
   return guarded([&] {
     need(plan && npasses, "null pointer");

@@ -142,6 +142,7 @@ template <typename Real> struct LineIO {
   int64_t is, os;
   int32_t kind, n, inverse;
   Real scale;
+  Real hw;  // interior half-spectrum weight (1: none)
 
   __device__ __forceinline__ C load(int32_t pos) const {
     C v;
@@ -156,7 +157,12 @@ template <typename Real> struct LineIO {
       } else {
         v = cconj(reinterpret_cast<const C*>(ib)[(n - pos) * is]);
       }
-      if (pos == 0 || 2 * pos == n) v.y = Real(0);
+      if (pos == 0 || 2 * pos == n) {
+        v.y = Real(0);
+      } else {
+        v.x *= hw;
+        v.y *= hw;
+      }
     }
     return inverse ? cconj(v) : v;
   }
@@ -167,6 +173,10 @@ template <typename Real> struct LineIO {
     if (kind == kC2C) {
       reinterpret_cast<C*>(ob)[pos * os] = v;
     } else if (kind == kR2C) {
+      if (pos != 0 && 2 * pos != n) {
+        v.x *= hw;
+        v.y *= hw;
+      }
       if (2 * pos <= n) reinterpret_cast<C*>(ob)[pos * os] = v;
     } else {
       reinterpret_cast<Real*>(ob)[pos * os] = v.x;
@@ -203,6 +213,7 @@ __device__ __forceinline__ LineIO<Real> make_line_io(const LinePass& p, int32_t 
   io.n = p.n;
   io.inverse = p.inverse;
   io.scale = static_cast<Real>(p.scale);
+  io.hw = p.herm_weight != 0.0 ? static_cast<Real>(p.herm_weight) : Real(1);
   return io;
 }
 
@@ -400,6 +411,7 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
       io.n = n;
       io.inverse = p.inverse;
       io.scale = static_cast<Real>(p.scale);
+      io.hw = p.herm_weight != 0.0 ? static_cast<Real>(p.herm_weight) : Real(1);
       if (p.kind == kC2R && pos >= n / 2 + 1) {
         idx[p.axis] = n - pos;
       } else {
@@ -413,7 +425,12 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
       if (p.kind == kC2R) {
         v = *reinterpret_cast<const C*>(io.ib);
         if (pos >= n / 2 + 1) v = cconj(v);
-        if (pos == 0 || 2 * pos == n) v.y = Real(0);
+        if (pos == 0 || 2 * pos == n) {
+          v.y = Real(0);
+        } else {
+          v.x *= io.hw;
+          v.y *= io.hw;
+        }
         if (p.inverse) v = cconj(v);
       } else {
         v = io.load(0);
@@ -465,9 +482,16 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
     io.n = n;
     io.inverse = p.inverse;
     io.scale = static_cast<Real>(p.scale);
+    io.hw = Real(1);
     io.ob = element_ptr(p.out, idx, oes, p.axis, st);
     io.os = 0;
-    io.store(0, bufA[e]);
+    // store(0, .) sees position 0, so the interior weight is applied here
+    C v = bufA[e];
+    if (p.kind == kR2C && pos != 0 && 2 * pos != n && p.herm_weight != 0.0) {
+      v.x *= static_cast<Real>(p.herm_weight);
+      v.y *= static_cast<Real>(p.herm_weight);
+    }
+    io.store(0, v);
   }
 }
 

@@ -31,6 +31,10 @@ struct LinePass {
   int32_t values_per_thread;            // 0: default (8); compile-time kernels exist for 4, 8, 16
   const void* tw;                       // exp(-2 pi i e / n), e in [0, n), Real precision
   double scale;
+  // Weight of the interior half-spectrum entries (0 < k < n/2) of an R2C /
+  // C2R line, applied to the values a C2R line loads or an R2C line stores;
+  // 0 means 1.  Adjoint transforms use 1/2 (C2R) and 2 (R2C).
+  double herm_weight;
 };
 
 // Element-wise permuted copy between two distributions of the same logical

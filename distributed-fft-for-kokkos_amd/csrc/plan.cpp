@@ -301,6 +301,7 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   p.n = s.n;
   p.inverse = inverse ? 1 : 0;
   p.scale = scale;
+  p.herm_weight = s.kind == kC2C ? 0.0 : adj_herm_;
   p.tw = twiddles(s.n);
   p.lines_per_block = s.lines_per_block;
   p.values_per_thread = s.values_per_thread;
@@ -336,7 +337,7 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
 
 void Plan::run_fused(void* const* in, void* const* out, bool forward, hipStream_t stream) {
   const auto& passes = forward ? fwd_ : bwd_;
-  const double scale = norm_scale(forward);
+  const double scale = exec_scale(forward);
   // first and last descriptors
   DDesc src = forward ? user_desc(gin_, in_topo_, in_tl_, in) : user_desc(gout_, out_topo_, out_tl_, in);
   DDesc dst = forward ? user_desc(gout_, out_topo_, out_tl_, out) : user_desc(gin_, in_topo_, in_tl_, out);
@@ -427,6 +428,29 @@ void Plan::execute(void* const* in, void* const* out, int direction, hipStream_t
   if (total_size(gin_) == 0) return;
   if (d_.exec_mode == DFFT_EXEC_BLOCKWALK) run_blockwalk(in, out, forward, stream);
   else run_fused(in, out, forward, stream);
+}
+
+// The adjoint of a transform is the opposite-direction pass sequence with
+// the transform's own scale: (s F)^* = s n F^{-1} for C2C.  For R2C the
+// half-spectrum side counts each stored entry once, so
+//   R2C^* y = C2R(w y),  w = 1 at k = 0 and k = n/2, 1/2 in between,
+//   C2R^* x = w' R2C(x), w' = 1 at k = 0 and k = n/2, 2 in between,
+// with the weight applied in the R2C / C2R line pass (it commutes with the
+// transforms along the other axes).
+void Plan::execute_adjoint(void* const* in, void* const* out, int direction, hipStream_t stream) {
+  if (direction != DFFT_FORWARD && direction != DFFT_BACKWARD)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "Invalid FFT direction specified.");
+  const bool forward = direction == DFFT_FORWARD;
+  struct Reset {
+    Plan* p;
+    ~Reset() {
+      p->adj_scale_ = -1.0;
+      p->adj_herm_ = 0.0;
+    }
+  } reset{this};
+  adj_scale_ = norm_scale(forward);
+  adj_herm_ = real_ ? (forward ? 0.5 : 2.0) : 0.0;
+  execute(in, out, forward ? DFFT_BACKWARD : DFFT_FORWARD, stream);
 }
 
 void Plan::mark(hipStream_t s) {
@@ -653,7 +677,7 @@ void Plan::run_blockwalk(void* const* in, void* const* out, bool forward, hipStr
       if (blocks[k].block_type == BlockType::Transpose) ++c;
     return c;
   };
-  double scale = norm_scale(forward);
+  double scale = exec_scale(forward);
   bool scaled = false;
   Vec ident = identity_map(rank_);
 

@@ -33,6 +33,9 @@ class Plan {
   Plan& operator=(const Plan&) = delete;
 
   void execute(void* const* in, void* const* out, int direction, hipStream_t stream);
+  // adjoint of execute(direction) under the real inner product Re sum conj(a) b
+  // of each side: in has the shape of that direction's output, out of its input
+  void execute_adjoint(void* const* in, void* const* out, int direction, hipStream_t stream);
   // execute with a HIP event pair around every launch; durations in ms
   std::vector<float> execute_timed(void* const* in, void* const* out, int direction, hipStream_t stream);
   int num_passes(int direction) const;
@@ -65,6 +68,7 @@ class Plan {
   void run_blockwalk(void* const* in, void* const* out, bool forward, hipStream_t s);
   const void* twiddles(int n) const;
   double norm_scale(bool forward) const;
+  double exec_scale(bool forward) const { return adj_scale_ >= 0.0 ? adj_scale_ : norm_scale(forward); }
 
   dfft_desc d_{};
   TopologyType kind_ = TopologyType::Shared;
@@ -84,6 +88,10 @@ class Plan {
   std::map<int, void*> tw_;
   std::vector<PassSpec> fwd_, bwd_;
   int chunk_[2] = {0, 0};  // planes per chunk, forward / backward (0 = no chunking)
+  // set only while execute_adjoint runs: scale of the adjoint direction and
+  // the interior half-spectrum weight of its R2C / C2R line pass
+  double adj_scale_ = -1.0;
+  double adj_herm_ = 0.0;
   // timing hooks (set only by execute_timed)
   std::vector<hipEvent_t>* timing_ = nullptr;
   void mark(hipStream_t s);

@@ -157,7 +157,8 @@ class NsDesc(ctypes.Structure):
 EXPORTED = [
     "dfft_last_error", "dfft_version", "dfft_plan_create", "dfft_plan_destroy", "dfft_plan_name",
     "dfft_plan_local_extents", "dfft_plan_num_blocks", "dfft_plan_block_info",
-    "dfft_plan_max_buffer_size", "dfft_plan_workspace_bytes", "dfft_execute", "dfft_plan_num_passes",
+    "dfft_plan_max_buffer_size", "dfft_plan_workspace_bytes", "dfft_execute", "dfft_execute_adjoint",
+    "dfft_plan_num_passes",
     "dfft_execute_timed", "dfft_plan_set_option", "dfft_plan_get_option", "dfft_plan_autotune", "dfft_pack",
     "dfft_unpack", "dfft_all2all", "dfft_exchange", "dfft_get_all_slab_topologies",
     "dfft_get_all_pencil_topologies", "dfft_block_analyses", "dfft_ns_create", "dfft_ns_destroy",
@@ -195,6 +196,7 @@ def load_library(path: os.PathLike | None = None):
         "dfft_plan_max_buffer_size": ([ctypes.c_void_p, szp], ctypes.c_int),
         "dfft_plan_workspace_bytes": ([ctypes.c_void_p, szp], ctypes.c_int),
         "dfft_execute": ([ctypes.c_void_p, vpp, vpp, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+        "dfft_execute_adjoint": ([ctypes.c_void_p, vpp, vpp, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
         "dfft_plan_num_passes": ([ctypes.c_void_p, ctypes.c_int, ip], ctypes.c_int),
         "dfft_execute_timed": ([ctypes.c_void_p, vpp, vpp, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.POINTER(ctypes.c_float), ctypes.c_int, ip], ctypes.c_int),
@@ -445,6 +447,11 @@ class Plan:
     def _execute(self, ins, outs, direction, stream=None):
         _check(self._lib.dfft_execute(self._plan, _ptrs(ins), _ptrs(outs), int(direction),
                                       _stream_handle(stream)))
+
+    def execute_adjoint(self, ins, outs, direction, stream=None):
+        """Adjoint of ``execute(direction)``: ``ins`` are shaped like that direction's outputs."""
+        _check(self._lib.dfft_execute_adjoint(self._plan, _ptrs(ins), _ptrs(outs), int(direction),
+                                              _stream_handle(stream)))
 
     def num_passes(self, direction):
         v = ctypes.c_int()

@@ -123,6 +123,14 @@ int dfft_plan_workspace_bytes(const dfft_plan* plan, size_t* bytes);
  * backward reads `in` (the P complex spectra) and writes `out` (P arrays of
  * the plan's input shape).  Inputs are not modified. */
 int dfft_execute(dfft_plan* plan, void* const* in, void* const* out, int direction, void* stream);
+/* Adjoint of dfft_execute(direction) under the plain real inner product
+ * Re sum conj(a) b of each side (every stored half-spectrum entry counted
+ * once): `in` has the shape of that direction's output, `out` of its input.
+ * C2C: the opposite transform with this direction's scale; R2C forward:
+ * C2R with the interior entries (0 < k < n/2) halved; C2R backward: R2C
+ * with them doubled.  No counterpart in the reference -- for gradient-based
+ * callers (e.g. the torch.autograd test harness). */
+int dfft_execute_adjoint(dfft_plan* plan, void* const* in, void* const* out, int direction, void* stream);
 
 /* Number of kernel launches (line passes, remaps) one execute of `direction` issues. */
 int dfft_plan_num_passes(const dfft_plan* plan, int direction, int* npasses);
