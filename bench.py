@@ -11,6 +11,18 @@ better).  `roofline` is the dominant kernel's algorithmic bytes (input +
 output array of the pass) over its average duration, timed with HIP events
 on the launch stream; `cpu_baseline` times the CPU restatement of the plan
 (oracle, single thread for the FFT lines) on a bounded sample and scales it.
+
+Further sections of the same line:
+* ns: Taylor-Green vortex 256^3 float64, Re = 1600, dt = 1e-3, 2x2 virtual
+  ranks, divergence form (3 velocity + 6 product fields per RK stage, each set
+  in one batched transform), steps replayed from a captured hipGraph;
+* ns_adjoint: 20 RK4 steps at 128^3 from the Taylor-Green field plus
+  N(0, 1e-2) noise, checkpoint every 5 steps: gradient of the final kinetic
+  energy; steps/s = 20 / (forward + recomputation + adjoint time);
+* mixed_batch: 64 C2C transforms, N cycling through {96, 128, 160}^3
+  complex64, forward + backward, one grouped launch per pass, hipGraph;
+* precision: FP32 against FP64 (rel-L2) for 512^3 C2C and 1024^3 R2C on
+  U[-1,1] inputs, and against the exact spectrum of sin(k.x), k = (3, 5, 7).
 """
 
 import argparse
@@ -73,7 +85,7 @@ def pass_times(dfft, plan, ins, outs, backs, reps):
 
 def cpu_baseline():
     """Oracle (CPU restatement of the slab plan, P=4 virtual ranks) on a
-    bounded sample: one 128^3 complex128 forward+backward, scaled to 512^3 by
+    bounded sample: one 256^3 complex128 forward+backward, scaled to 512^3 by
     the N log N operation count."""
     import numpy as np
     from oracle import plan_exec as PE
@@ -95,6 +107,151 @@ def cpu_baseline():
                       f"scaled x{scale:.1f} by N log N"}
 
 
+def _sync_ms(torch, fn, reps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def bench_ns(torch, steps):
+    import dfft_ns
+    n = 256
+    s = dfft_ns.NavierStokes(n, 2, 2, lx=2.0, nu=1.0 / 1600.0, dt=1e-3, form=dfft_ns.DIVERGENCE)
+    s.init_taylor_green(1.0)
+    e0 = s.energy()
+    s.step(2, use_graph=True)  # eager first step + capture
+    ms = _sync_ms(torch, lambda: s.step(1, use_graph=True), steps)
+    e1 = s.energy()
+    s.destroy()
+    fields = 9 * 4  # transforms per step: (3 backward + 6 forward) x 4 stages
+    return {"workload": "Taylor-Green 256^3 float64, 2x2 virtual ranks, RK4, divergence form, hipGraph",
+            "ms_per_step": round(ms, 4), "steps_per_s": round(1e3 / ms, 3),
+            "transforms_per_step": fields, "energy_t0": e0, "energy_after": e1}
+
+
+def bench_ns_adjoint(torch, dev):
+    import dfft_ns
+    n, nsteps, ck = 128, 20, 5
+    # Taylor-Green on [0, 2 pi)^3 (u = sin x cos y cos z, v = -cos x sin y cos z, w = 0) + N(0, 1e-2)
+    x = torch.arange(n, dtype=torch.float64, device=dev) * (2 * math.pi / n)
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    gen = torch.Generator(device=dev).manual_seed(5)
+    u0 = torch.stack([torch.sin(X) * torch.cos(Y) * torch.cos(Z), -torch.cos(X) * torch.sin(Y) * torch.cos(Z),
+                      torch.zeros_like(X)])
+    u0 = (u0 + 1e-2 * torch.randn(u0.shape, generator=gen, device=dev, dtype=torch.float64)).contiguous()
+    s = dfft_ns.NavierStokes(n, 1, 1, lx=2.0, nu=1.0 / 1600.0, dt=1e-3, form=dfft_ns.DIVERGENCE)
+    u0d = [u0]
+    s.gradient(2, u0d, checkpoint_every=1)  # warm-up (allocations, kernel setup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    j, grad = s.gradient(nsteps, u0d, checkpoint_every=ck)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    fwd_ms = _sync_ms(torch, lambda: s.final_energy(nsteps, u0d), 2)
+    s.destroy()
+    return {"workload": "adjoint of 20 RK4 steps, Taylor-Green 128^3 float64 + N(0,1e-2) noise, "
+                        "checkpoint every 5", "steps_per_s": round(nsteps / t_adj, 3),
+            "gradient_s": round(t_adj, 4), "forward_only_steps_per_s": round(nsteps / (fwd_ms * 1e-3), 3),
+            "adjoint_over_forward": round(t_adj / (fwd_ms * 1e-3), 3), "J": j}
+
+
+def bench_mixed_batch(torch, dfft, dev, steps, warmup):
+    sizes = [(96, 128, 160)[i % 3] for i in range(64)]
+    gen = torch.Generator(device=dev).manual_seed(1234)
+    ins, outs = [], []
+    for n in sizes:
+        re = torch.rand((n, n, n), generator=gen, device=dev, dtype=torch.float32) * 2 - 1
+        im = torch.rand((n, n, n), generator=gen, device=dev, dtype=torch.float32) * 2 - 1
+        ins.append(torch.complex(re, im))
+        outs.append(torch.empty((n, n, n), dtype=torch.complex64, device=dev))
+    backs = [torch.empty_like(t) for t in ins]
+    g = dfft.Group(ins, outs)
+
+    def rt():
+        g.execute(ins, outs, dfft.Direction.forward, use_graph=True)
+        g.execute(outs, backs, dfft.Direction.backward, use_graph=True)
+
+    for _ in range(max(1, warmup)):
+        rt()
+    ms = _sync_ms(torch, rt, steps)
+    err = max(float(torch.linalg.vector_norm(b - a) / torch.linalg.vector_norm(a)) for a, b in zip(ins, backs))
+    vol = sum(n ** 3 for n in sizes)
+    launches = g.num_launches(dfft.Direction.forward) + g.num_launches(dfft.Direction.backward)
+    pass_bytes = 2 * vol * 8
+    g.destroy()
+    return {"workload": "64 C2C transforms, N in {96,128,160}^3 (22/21/21) complex64, fwd+bwd, grouped, hipGraph",
+            "ms": round(ms, 4), "launches": launches,
+            "avg_pass_GBs": round(pass_bytes * launches / (ms * 1e-3) / 1e9, 1),
+            "roundtrip_rel_l2_max": err}
+
+
+def bench_precision(torch, dfft, dev):
+    """FP32 vs FP64 (both on the GPU; the FP64 path agrees with the CPU oracle to 1e-12
+    wherever the oracle runs) and vs the exact spectrum of a single mode."""
+    out = {}
+    for tag, n, real in (("c2c_512", 512, False), ("r2c_1024", 1024, True)):
+        gin = [n] * 3
+        res = {}
+        for field in ("uniform", "sin_k357"):
+            spectra = {}
+            for fp64 in (True, False):
+                rdt = torch.float64 if fp64 else torch.float32
+                cdt = torch.complex128 if fp64 else torch.complex64
+                gen = torch.Generator(device=dev).manual_seed(11)
+                ins, outs = [], []
+                for r in range(P):
+                    ext, st = dfft.compute_local_extents_and_starts(gin, [P, 1, 1], r)
+                    if field == "uniform":
+                        re = torch.rand(ext, generator=gen, device=dev, dtype=torch.float64) * 2 - 1
+                        x = re if real else torch.complex(re, torch.rand(ext, generator=gen, device=dev,
+                                                                        dtype=torch.float64) * 2 - 1)
+                    else:
+                        ii = [torch.arange(st[a], st[a] + ext[a], device=dev, dtype=torch.float64)
+                              for a in range(3)]
+                        X, Y, Z = torch.meshgrid(*ii, indexing="ij")
+                        x = torch.sin(2 * math.pi * (3 * X + 5 * Y + 7 * Z) / n)
+                        if not real:
+                            x = torch.complex(x, torch.zeros_like(x))
+                    ins.append(x.to(rdt if real else cdt).contiguous())
+                    oe, _ = dfft.compute_local_extents_and_starts([n, n, n // 2 + 1] if real else gin, [1, P, 1], r)
+                    outs.append(torch.empty(oe, dtype=cdt, device=dev))
+                gout = [n, n, n // 2 + 1] if real else gin
+                plan = dfft.Plan(ins, outs, [0, 1, 2], [P, 1, 1], [1, P, 1], global_in_extents=gin,
+                                 global_out_extents=gout)
+                dfft.execute(plan, ins, outs, dfft.Direction.forward)
+                torch.cuda.synchronize()
+                spectra[fp64] = [o.to(torch.complex128) for o in outs]
+                del plan, ins, outs
+                torch.cuda.empty_cache()
+            num = sum(float(torch.linalg.vector_norm(a - b)) ** 2 for a, b in zip(spectra[False], spectra[True]))
+            den = sum(float(torch.linalg.vector_norm(b)) ** 2 for b in spectra[True])
+            res[field] = {"fp32_vs_fp64": math.sqrt(num / den)}
+            if field == "sin_k357":
+                # exact: n^3/(2i) at k and -n^3/(2i) at -k (the R2C half holds only k = (3,5,7))
+                for fp64 in (True, False):
+                    num = den = 0.0
+                    for r, blk in enumerate(spectra[fp64]):
+                        ext, st = dfft.compute_local_extents_and_starts(gout, [1, P, 1], r)
+                        want = torch.zeros_like(blk)
+                        for sgn in (1, -1):
+                            k = [(sgn * 3) % n, (sgn * 5) % n, (sgn * 7) % n]
+                            if real and sgn < 0:
+                                continue
+                            loc = [k[a] - st[a] for a in range(3)]
+                            if all(0 <= loc[a] < ext[a] for a in range(3)):
+                                want[loc[0], loc[1], loc[2]] = complex(0.0, -sgn * n ** 3 / 2)
+                        num += float(torch.linalg.vector_norm(blk - want)) ** 2
+                        den += float(torch.linalg.vector_norm(want)) ** 2
+                    res[field]["fp64_vs_exact" if fp64 else "fp32_vs_exact"] = math.sqrt(num / den)
+            del spectra
+            torch.cuda.empty_cache()
+        out[tag] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +261,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the plan autotuner")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
                     help="run one precision only (profiling runs)")
+    ap.add_argument("--sections", default="fft,ns,adjoint,batch,precision",
+                    help="comma list of: fft (headline), ns, adjoint, batch, precision")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -183,6 +342,23 @@ def main():
                     "fp32": {"fwd": results["fp32"]["fwd_pass_ms"], "bwd": results["fp32"]["bwd_pass_ms"]}},
         "roofline": roofline,
     }
+    sections = set(args.sections.split(","))
+    if "ns" in sections:
+        log("[bench] NS Taylor-Green 256^3 ...")
+        line["ns"] = bench_ns(torch, max(3, args.steps))
+        torch.cuda.empty_cache()
+    if "adjoint" in sections:
+        log("[bench] NS adjoint 128^3, 20 steps ...")
+        line["ns_adjoint"] = bench_ns_adjoint(torch, dev)
+        torch.cuda.empty_cache()
+    if "batch" in sections:
+        log("[bench] mixed batch of 64 transforms ...")
+        line["mixed_batch"] = bench_mixed_batch(torch, dfft, dev, args.steps, args.warmup)
+        torch.cuda.empty_cache()
+    if "precision" in sections:
+        log("[bench] precision FP32 vs FP64 ...")
+        line["precision"] = bench_precision(torch, dfft, dev)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline()

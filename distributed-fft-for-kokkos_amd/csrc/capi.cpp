@@ -7,6 +7,7 @@
 
 #include "dfft.h"
 #include "fft_kernels.hpp"
+#include "group.hpp"
 #include "navier_stokes.hpp"
 #include "plan.hpp"
 #include "planner.hpp"
@@ -17,6 +18,10 @@ struct dfft_plan {
 
 struct dfft_ns {
   dfft::NavierStokes* impl;
+};
+
+struct dfft_group {
+  dfft::Group* impl;
 };
 
 namespace {
@@ -355,6 +360,44 @@ int dfft_exchange(void* const* in, void* const* out, int rank, const size_t* glo
         if (in[r] == out[q] && in[r] != nullptr)
           throw dfft::Error(DFFT_ERROR_ALIAS, "input and output local arrays must not alias");
     hip_ok(dfft::launch_remap(ra, elem_size, static_cast<hipStream_t>(stream)), "exchange");
+  });
+}
+
+int dfft_group_create(int count, const dfft_desc* descs, dfft_group** group) {
+  return guarded([&] {
+    need(descs && group, "null pointer");
+    *group = nullptr;
+    auto* h = new dfft_group{nullptr};
+    try {
+      h->impl = new dfft::Group(count, descs);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *group = h;
+  });
+}
+
+int dfft_group_destroy(dfft_group* group) {
+  return guarded([&] {
+    if (!group) return;
+    delete group->impl;
+    delete group;
+  });
+}
+
+int dfft_group_execute(dfft_group* group, void* const* in, void* const* out, int direction, int use_graph,
+                       void* stream) {
+  return guarded([&] {
+    need(group, "null group");
+    group->impl->execute(in, out, direction, use_graph != 0, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_group_num_launches(const dfft_group* group, int direction, int* nlaunches) {
+  return guarded([&] {
+    need(group && nlaunches, "null pointer");
+    *nlaunches = group->impl->num_launches(direction);
   });
 }
 

@@ -27,15 +27,20 @@ struct FPlan {
 
 constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
 
-// Radix-8 stages first, then one 4 or 2, then 3s and 5s.  R (values per
-// thread) is the least common multiple of the radices, doubled while below
-// `rmin` and still a divisor of n.
+// Power-of-two stages first, then 3s and 5s.  R (values per thread) is the
+// least common multiple of the radices, doubled while below `rmin` and still
+// a divisor of n.  With an odd factor the power-of-two part uses radix 4 (not
+// 8) so R stays small: 160 = 4*4*2*5 holds 20 values per thread, not 40.
 constexpr FPlan make_plan(int n, int rmin = 1) {
   FPlan p;
   if (n <= 1) return p;
   int m = n;
-  while (m % 8 == 0) { p.rad[p.nst++] = 8; m /= 8; }
-  if (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
+  int odd = n;
+  while (odd % 2 == 0) odd /= 2;
+  if (odd == 1) {
+    while (m % 8 == 0) { p.rad[p.nst++] = 8; m /= 8; }
+  }
+  while (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
   if (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
   while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
   while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; }
@@ -317,14 +322,15 @@ template <typename Real, int N, int RM, int S> struct StageLoop {
   }
 };
 
+// One block's share of a line pass: lines [blk*B, blk*B + B) of p, T = N/R
+// lanes per line.  Lanes beyond B*T (a block shape shared by several line
+// lengths) run the stages on zeros in a spare LDS slot (index B) so every
+// lane reaches every barrier; they load and store nothing.
 template <typename Real, int N, int RM>
-__global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
+__device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, int B, cplx_t<Real>* lds) {
   using C = cplx_t<Real>;
   using Cfg = FastCfg<Real, N, RM>;
   constexpr int R = Cfg::R, T = Cfg::T, NST = Cfg::NST;
-  extern __shared__ unsigned char smem_raw[];
-  C* lds = reinterpret_cast<C*>(smem_raw);
-  const int B = p.lines_per_block;
   const int tid = threadIdx.x;
   int b, t;
   if (p.lines_fast) {
@@ -334,8 +340,13 @@ __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
     t = tid % T;
     b = tid / T;
   }
-  const int32_t line = blockIdx.x * B + b;
-  const bool valid = line < p.nlines;
+  const bool active = tid < B * T;
+  if (!active) {
+    b = B;
+    t = 0;
+  }
+  const int32_t line = blk * B + b;
+  const bool valid = active && line < p.nlines;
   const double2* tw = reinterpret_cast<const double2*>(p.tw);
   C* line_lds = lds + b * Cfg::LDSN;
 
@@ -373,6 +384,44 @@ __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   if (valid) {
 #pragma unroll
     for (int j = 0; j < R; ++j) io.store(t + j * T, v[j]);
+  }
+}
+
+template <typename Real, int N, int RM>
+__global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
+  extern __shared__ unsigned char smem_raw[];
+  fast_line_body<Real, N, RM>(p, blockIdx.x, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
+}
+
+// ---------------------------------------------------------------------------
+// Grouped line pass: the same pass of many independent transforms (lengths
+// from kGroupSizes, any mix) in one launch.  Virtual blocks are numbered
+// transform by transform (first_block[i] .. first_block[i+1]); a grid of at
+// most a few blocks per CU walks them in a block-stride loop.  Every block
+// has kGroupThreads lanes; transform i uses items[i].lines_per_block lines.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* __restrict__ items,
+                                                                const int32_t* __restrict__ first_block,
+                                                                int32_t count, int32_t total_blocks) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx_t<Real>* lds = reinterpret_cast<cplx_t<Real>*>(smem_raw);
+  int32_t i = 0;
+  for (int32_t vb = blockIdx.x; vb < total_blocks; vb += gridDim.x) {
+    while (i + 1 < count && first_block[i + 1] <= vb) ++i;  // vb only grows: a forward scan
+    const LinePass& p = items[i];
+    const int32_t blk = vb - first_block[i];
+    switch (p.n) {
+      case 16: fast_line_body<Real, 16, 8>(p, blk, p.lines_per_block, lds); break;
+      case 32: fast_line_body<Real, 32, 8>(p, blk, p.lines_per_block, lds); break;
+      case 64: fast_line_body<Real, 64, 8>(p, blk, p.lines_per_block, lds); break;
+      case 96: fast_line_body<Real, 96, 8>(p, blk, p.lines_per_block, lds); break;
+      case 128: fast_line_body<Real, 128, 8>(p, blk, p.lines_per_block, lds); break;
+      case 160: fast_line_body<Real, 160, 8>(p, blk, p.lines_per_block, lds); break;
+      case 256: fast_line_body<Real, 256, 8>(p, blk, p.lines_per_block, lds); break;
+      default: break;
+    }
+    __syncthreads();  // the next virtual block reuses the LDS lines
   }
 }
 
@@ -598,6 +647,61 @@ hipError_t dispatch(const LinePass& p, hipStream_t s) {
 }
 
 }  // namespace
+
+namespace {
+// lines per block: as many as the block's lanes hold, capped so the lines
+// plus the spare slot fit the 64 KB default dynamic LDS
+template <typename Real, int N>
+void group_shape_of(int* B, size_t* lds) {
+  using Cfg = FastCfg<Real, N, 8>;
+  const size_t line_bytes = static_cast<size_t>(Cfg::LDSN) * sizeof(cplx_t<Real>);
+  int b = kGroupThreads / Cfg::T;
+  while (b > 1 && static_cast<size_t>(b + 1) * line_bytes > kGroupLds) --b;
+  *B = b;
+  *lds = static_cast<size_t>(b + 1) * line_bytes;
+}
+template <typename Real>
+bool group_shape(int n, int* B, size_t* lds) {
+  switch (n) {
+    case 16: group_shape_of<Real, 16>(B, lds); return true;
+    case 32: group_shape_of<Real, 32>(B, lds); return true;
+    case 64: group_shape_of<Real, 64>(B, lds); return true;
+    case 96: group_shape_of<Real, 96>(B, lds); return true;
+    case 128: group_shape_of<Real, 128>(B, lds); return true;
+    case 160: group_shape_of<Real, 160>(B, lds); return true;
+    case 256: group_shape_of<Real, 256>(B, lds); return true;
+    default: return false;
+  }
+}
+}  // namespace
+
+bool group_line_shape(int n, bool fp64, int* lines_per_block, size_t* lds_bytes) {
+  return fp64 ? group_shape<double>(n, lines_per_block, lds_bytes) : group_shape<float>(n, lines_per_block, lds_bytes);
+}
+
+int group_grid(size_t lds_bytes, bool fp64) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const void* f = fp64 ? reinterpret_cast<const void*>(&fft_group_pass<double>)
+                       : reinterpret_cast<const void*>(&fft_group_pass<float>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kGroupThreads, lds_bytes) != hipSuccess) return 0;
+  return cus * (per_cu > 0 ? per_cu : 1);
+}
+
+hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
+                             int grid, size_t lds_bytes, bool fp64, hipStream_t s) {
+  if (total_blocks <= 0 || count <= 0) return hipSuccess;
+  if (grid <= 0 || lds_bytes > kGroupLds) return hipErrorInvalidConfiguration;
+  const unsigned g = static_cast<unsigned>(grid < total_blocks ? grid : total_blocks);
+  if (fp64)
+    hipLaunchKernelGGL(fft_group_pass<double>, dim3(g), dim3(kGroupThreads), lds_bytes, s, items, first_block, count,
+                       total_blocks);
+  else
+    hipLaunchKernelGGL(fft_group_pass<float>, dim3(g), dim3(kGroupThreads), lds_bytes, s, items, first_block, count,
+                       total_blocks);
+  return hipGetLastError();
+}
 
 bool has_fast_line_kernel(int n) {
   switch (n) {

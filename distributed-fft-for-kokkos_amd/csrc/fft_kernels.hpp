@@ -48,6 +48,17 @@ struct RemapArgs {
   FastDiv iter_div[kMaxDim];
 };
 
+// Grouped line pass (one launch for the same pass of many transforms, line
+// lengths 16, 32, 64, 96, 128, 160, 256 in any mix).
+constexpr int kGroupThreads = 256;
+constexpr size_t kGroupLds = 64 * 1024;  // dynamic LDS of a grouped launch, at most
+// lines per block and dynamic LDS bytes of the grouped kernel for length n;
+// false if n has no grouped kernel
+bool group_line_shape(int n, bool fp64, int* lines_per_block, size_t* lds_bytes);
+// grid of the persistent grouped launch: resident blocks of the whole device
+int group_grid(size_t lds_bytes, bool fp64);
+hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
+                             int grid, size_t lds_bytes, bool fp64, hipStream_t s);
 bool has_fast_line_kernel(int n);
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
 hipError_t launch_remap(const RemapArgs& a, int esize, hipStream_t s);

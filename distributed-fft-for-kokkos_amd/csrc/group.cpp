@@ -1,0 +1,334 @@
+#include "group.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "plan.hpp"
+
+namespace dfft {
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(DFFT_ERROR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// one single-GPU LayoutRight array
+DDesc array_desc(const Vec& ext, void* p) {
+  DDesc d{};
+  d.ndim = static_cast<int32_t>(ext.size());
+  d.layout_left = 0;
+  for (std::size_t a = 0; a < ext.size(); ++a) {
+    d.ext[a] = static_cast<int32_t>(ext[a]);
+    d.ntopo[a] = 1;
+    d.rstride[a] = 0;
+    d.base[a] = static_cast<int32_t>(ext[a]);
+    d.rem[a] = 0;
+    d.map[a] = static_cast<int32_t>(a);
+    d.div_big[a] = FastDiv(static_cast<uint32_t>(ext[a] + 1));
+    d.div_small[a] = FastDiv(static_cast<uint32_t>(std::max<std::size_t>(ext[a], 1)));
+  }
+  d.ptr[0] = static_cast<char*>(p);
+  return d;
+}
+
+std::size_t volume(const Vec& v) {
+  std::size_t n = 1;
+  for (auto e : v) n *= e;
+  return n;
+}
+
+}  // namespace
+
+Group::Group(int count, const dfft_desc* descs) {
+  if (count < 1 || count > 4096) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "group size must be in [1, 4096]");
+  if (!descs) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null descriptor array");
+  check(hipGetDevice(&device_), "hipGetDevice");
+  fp64_ = descs[0].precision == DFFT_FP64;
+  try {
+    for (int i = 0; i < count; ++i) {
+      const dfft_desc& d = descs[i];
+      const std::string who = "group member " + std::to_string(i) + ": ";
+      if (d.rank < 1 || d.rank > DFFT_MAX_RANK) throw Error(DFFT_ERROR_INVALID_ARGUMENT, who + "bad rank");
+      if (d.fft_rank < 1 || d.fft_rank > 3 || d.fft_rank > d.rank)
+        throw Error(DFFT_ERROR_INVALID_ARGUMENT, who + "bad number of FFT axes");
+      if ((d.precision == DFFT_FP64) != fp64_)
+        throw Error(DFFT_ERROR_MISMATCH, who + "all members must share one precision");
+      if (d.transform != DFFT_C2C && d.transform != DFFT_R2C)
+        throw Error(DFFT_ERROR_INVALID_ARGUMENT, who + "bad transform type");
+      if (d.layout != DFFT_LAYOUT_RIGHT) throw Error(DFFT_ERROR_UNSUPPORTED, who + "members must be LayoutRight");
+      if (d.norm < DFFT_NORM_BACKWARD || d.norm > DFFT_NORM_NONE)
+        throw Error(DFFT_ERROR_INVALID_ARGUMENT, who + "invalid normalization");
+      if (d.nranks != 1) throw Error(DFFT_ERROR_INVALID_TOPOLOGY, who + "members are single-GPU transforms");
+      Member m;
+      m.rank = d.rank;
+      m.real = d.transform == DFFT_R2C;
+      for (int a = 0; a < d.rank; ++a) {
+        if (d.in_topology[a] != 1 || d.out_topology[a] != 1)
+          throw Error(DFFT_ERROR_INVALID_TOPOLOGY, who + "members are single-GPU transforms");
+        m.gin.push_back(d.in_extents[a]);
+        m.gout.push_back(d.out_extents[a]);
+      }
+      std::vector<int> axes;
+      for (int k = 0; k < d.fft_rank; ++k) {
+        const int a = d.axes[k] < 0 ? d.axes[k] + d.rank : d.axes[k];
+        if (a < 0 || a >= d.rank || std::find(axes.begin(), axes.end(), a) != axes.end())
+          throw Error(DFFT_ERROR_INVALID_ARGUMENT, who + "bad FFT axes");
+        axes.push_back(a);
+      }
+      for (int a = 0; a < d.rank; ++a) {
+        std::size_t want = m.gin[a];
+        if (m.real && a == axes.back()) want = m.gin[a] / 2 + 1;
+        if (m.gout[a] != want || m.gin[a] == 0)
+          throw Error(DFFT_ERROR_INVALID_EXTENTS, who + "input and output extents do not match");
+      }
+      // forward: the R2C axis first, then the other axes innermost first;
+      // backward: the reverse, so a C2R pass comes last
+      std::vector<int> rest;
+      for (int a : axes)
+        if (!(m.real && a == axes.back())) rest.push_back(a);
+      std::sort(rest.begin(), rest.end(), [](int x, int y) { return x > y; });
+      if (m.real) m.fwd_axes.push_back(axes.back());
+      for (int a : rest) m.fwd_axes.push_back(a);
+      m.bwd_axes.assign(m.fwd_axes.rbegin(), m.fwd_axes.rend());
+      double n = 1.0;
+      for (int a : axes) n *= static_cast<double>(m.gin[a]);
+      switch (d.norm) {
+        case DFFT_NORM_BACKWARD: m.scale_f = 1.0; m.scale_b = 1.0 / n; break;
+        case DFFT_NORM_FORWARD: m.scale_f = 1.0 / n; m.scale_b = 1.0; break;
+        case DFFT_NORM_ORTHO: m.scale_f = m.scale_b = 1.0 / std::sqrt(n); break;
+        default: break;
+      }
+      for (int a : axes) {
+        const int len = static_cast<int>(m.gin[a]);
+        if (!tw_.count(len)) tw_[len] = upload_twiddles(len);
+      }
+      if (m.real && m.fwd_axes.size() > 1) {
+        const std::size_t bytes = volume(m.gout) * (fp64_ ? 16 : 8);
+        if (hipMalloc(&m.scratch, bytes) != hipSuccess)
+          throw Error(DFFT_ERROR_OUT_OF_MEMORY, "hipMalloc of a group member's scratch failed");
+      }
+      max_passes_ = std::max(max_passes_, static_cast<int>(m.fwd_axes.size()));
+      members_.push_back(m);
+    }
+  } catch (...) {
+    for (auto& m : members_)
+      if (m.scratch) (void)hipFree(m.scratch);
+    for (auto& kv : tw_) (void)hipFree(kv.second);
+    throw;
+  }
+}
+
+Group::~Group() {
+  for (auto& ds : dir_) {
+    if (ds.exec) (void)hipGraphExecDestroy(ds.exec);
+    if (ds.graph) (void)hipGraphDestroy(ds.graph);
+    for (auto* p : ds.items) (void)hipFree(p);
+    for (auto* p : ds.first) (void)hipFree(p);
+  }
+  if (graph_stream_) (void)hipStreamDestroy(graph_stream_);
+  for (auto& m : members_)
+    if (m.scratch) (void)hipFree(m.scratch);
+  for (auto& kv : tw_) (void)hipFree(kv.second);
+}
+
+// Line pass k of every member that has one (members in order).
+std::vector<LinePass> Group::build_pass(const std::vector<void*>& in, const std::vector<void*>& out, bool forward,
+                                        int k) const {
+  std::vector<LinePass> passes;
+  for (std::size_t i = 0; i < members_.size(); ++i) {
+    const Member& m = members_[i];
+    const std::vector<int>& order = forward ? m.fwd_axes : m.bwd_axes;
+    const int np = static_cast<int>(order.size());
+    if (k >= np) continue;
+    const bool first = k == 0, last = k + 1 == np;
+    const int axis = order[k];
+    int kind = kC2C;
+    if (m.real && forward && first) kind = kR2C;
+    if (m.real && !forward && last) kind = kC2R;
+    // source / destination arrays of this pass
+    DDesc src, dst;
+    if (forward) {
+      src = first ? array_desc(m.real ? m.gin : m.gout, in[i]) : array_desc(m.gout, out[i]);
+      dst = array_desc(m.gout, out[i]);
+    } else if (!m.real) {
+      src = array_desc(m.gout, first ? in[i] : out[i]);
+      dst = array_desc(m.gout, out[i]);
+    } else {
+      src = array_desc(m.gout, first ? in[i] : m.scratch);
+      dst = last ? array_desc(m.gin, out[i]) : array_desc(m.gout, m.scratch);
+    }
+    LinePass p{};
+    p.in = src;
+    p.out = dst;
+    p.axis = axis;
+    p.kind = kind;
+    p.n = static_cast<int32_t>(m.gin[axis]);
+    p.inverse = forward ? 0 : 1;
+    p.scale = last ? (forward ? m.scale_f : m.scale_b) : 1.0;
+    p.herm_weight = 0.0;
+    p.tw = tw_.at(p.n);
+    p.nbatch = 0;
+    int64_t nl = 1;
+    for (int a = 0; a < m.rank; ++a) {
+      if (a == axis) continue;
+      p.batch_axis[p.nbatch] = a;
+      p.batch_ext[p.nbatch] = src.ext[a];
+      p.batch_div[p.nbatch] = FastDiv(static_cast<uint32_t>(std::max(src.ext[a], 1)));
+      nl *= src.ext[a];
+      ++p.nbatch;
+    }
+    if (nl >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many lines in one pass");
+    p.nlines = static_cast<int32_t>(nl);
+    p.line_begin = 0;
+    p.lines_fast = axis != m.rank - 1 ? 1 : 0;
+    int B = 0;
+    std::size_t lds = 0;
+    p.values_per_thread = 8;
+    p.lines_per_block = group_line_shape(p.n, fp64_, &B, &lds) ? B : 0;
+    passes.push_back(p);
+  }
+  return passes;
+}
+
+void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, int dir, hipStream_t s) {
+  DirState& ds = dir_[dir];
+  check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  const std::size_t np = static_cast<std::size_t>(max_passes_);
+  ds.items.resize(np, nullptr);
+  ds.first.resize(np, nullptr);
+  ds.count.assign(np, 0);
+  ds.total.assign(np, 0);
+  ds.grid.assign(np, 0);
+  ds.lds.assign(np, 0);
+  ds.single.assign(np, {});
+  for (std::size_t k = 0; k < np; ++k) {
+    std::vector<LinePass> all = build_pass(in, out, dir == 0, static_cast<int>(k));
+    std::vector<LinePass> grouped;
+    std::vector<int32_t> first{0};
+    std::size_t lds = 0;
+    for (const LinePass& p : all) {
+      int B = 0;
+      std::size_t l = 0;
+      if (p.nlines == 0) continue;
+      if (group_line_shape(p.n, fp64_, &B, &l)) {
+        grouped.push_back(p);
+        const int64_t nb = first.back() + (static_cast<int64_t>(p.nlines) + B - 1) / B;
+        if (nb >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many blocks in one grouped pass");
+        first.push_back(static_cast<int32_t>(nb));
+        lds = std::max(lds, l);
+      } else {
+        LinePass q = p;
+        q.lines_per_block = 0;  // launcher heuristic
+        q.values_per_thread = 0;
+        ds.single[k].push_back(q);
+      }
+    }
+    ds.count[k] = static_cast<int>(grouped.size());
+    ds.total[k] = first.back();
+    ds.lds[k] = lds;
+    if (grouped.empty()) continue;
+    ds.grid[k] = group_grid(lds, fp64_);
+    if (ds.grid[k] <= 0) throw Error(DFFT_ERROR_HIP, "occupancy query of the grouped kernel failed");
+    if (!ds.items[k]) {
+      check(hipMalloc(&ds.items[k], sizeof(LinePass) * members_.size()), "hipMalloc");
+      check(hipMalloc(&ds.first[k], sizeof(int32_t) * (members_.size() + 1)), "hipMalloc");
+    }
+    check(hipMemcpy(ds.items[k], grouped.data(), sizeof(LinePass) * grouped.size(), hipMemcpyHostToDevice),
+          "hipMemcpy");
+    check(hipMemcpy(ds.first[k], first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  ds.in = in;
+  ds.out = out;
+  ds.built = true;
+}
+
+void Group::launch_all(int dir, hipStream_t s) {
+  const DirState& ds = dir_[dir];
+  for (int k = 0; k < max_passes_; ++k) {
+    if (ds.count[k] > 0)
+      check(launch_group_pass(ds.items[k], ds.first[k], ds.count[k], ds.total[k], ds.grid[k], ds.lds[k], fp64_, s),
+            "grouped line pass");
+    for (const LinePass& p : ds.single[k]) check(launch_line_pass(p, fp64_, s), "line pass");
+  }
+}
+
+int Group::num_launches(int direction) const {
+  (void)direction;
+  int n = 0;
+  for (int k = 0; k < max_passes_; ++k) {
+    bool grouped = false;
+    for (const Member& m : members_) {
+      if (k >= static_cast<int>(m.fwd_axes.size())) continue;
+      const std::vector<int>& order = direction == DFFT_FORWARD ? m.fwd_axes : m.bwd_axes;
+      int B = 0;
+      std::size_t l = 0;
+      if (group_line_shape(static_cast<int>(m.gin[order[k]]), fp64_, &B, &l)) grouped = true;
+      else ++n;
+    }
+    if (grouped) ++n;
+  }
+  return n;
+}
+
+void Group::execute(void* const* in, void* const* out, int direction, bool use_graph, hipStream_t s) {
+  if (!in || !out) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null pointer table");
+  if (direction != DFFT_FORWARD && direction != DFFT_BACKWARD)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "Invalid FFT direction specified.");
+  const int dir = direction == DFFT_FORWARD ? 0 : 1;
+  std::vector<void*> vin(in, in + members_.size()), vout(out, out + members_.size());
+  for (std::size_t i = 0; i < members_.size(); ++i) {
+    if (!vin[i] || !vout[i]) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null member array");
+    if (members_[i].real && vin[i] == vout[i])
+      throw Error(DFFT_ERROR_ALIAS, "an R2C member cannot run in place");
+  }
+  int cur = -1;
+  check(hipGetDevice(&cur), "hipGetDevice");
+  if (cur != device_) check(hipSetDevice(device_), "hipSetDevice");
+  DirState& ds = dir_[dir];
+  if (!ds.built || ds.in != vin || ds.out != vout) {
+    if (ds.exec) {
+      check(hipStreamSynchronize(graph_stream_), "hipStreamSynchronize");
+      (void)hipGraphExecDestroy(ds.exec);
+      ds.exec = nullptr;
+    }
+    if (ds.graph) {
+      (void)hipGraphDestroy(ds.graph);
+      ds.graph = nullptr;
+    }
+    upload(vin, vout, dir, s);
+  }
+  if (!use_graph) {
+    launch_all(dir, s);
+    return;
+  }
+  if (!ds.exec) {
+    // eager run for this call; the capture below records the same launches
+    launch_all(dir, s);
+    if (!graph_stream_) check(hipStreamCreate(&graph_stream_), "hipStreamCreate");
+    check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    check(hipStreamBeginCapture(graph_stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      launch_all(dir, graph_stream_);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(graph_stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    check(hipStreamEndCapture(graph_stream_, &ds.graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&ds.exec, ds.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    return;
+  }
+  hipEvent_t ev;
+  check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+  check(hipEventRecord(ev, s), "hipEventRecord");
+  check(hipStreamWaitEvent(graph_stream_, ev, 0), "hipStreamWaitEvent");
+  check(hipGraphLaunch(ds.exec, graph_stream_), "hipGraphLaunch");
+  check(hipEventRecord(ev, graph_stream_), "hipEventRecord");
+  check(hipStreamWaitEvent(s, ev, 0), "hipStreamWaitEvent");
+  (void)hipEventDestroy(ev);
+}
+
+}  // namespace dfft

@@ -1,0 +1,68 @@
+// A group of independent single-GPU transforms of possibly different
+// extents executed together: line pass k of every member is ONE grouped
+// kernel launch (the reference would loop execute() over separate plans,
+// Plan.hpp:212-238; its batched NS example batches only equal-size fields
+// through a leading axis, navier-stokes-MPI-batched/main.cpp:102-107).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "dfft.h"
+#include "fft_kernels.hpp"
+#include "planner.hpp"
+
+namespace dfft {
+
+class Group {
+ public:
+  Group(int count, const dfft_desc* descs);
+  ~Group();
+  Group(const Group&) = delete;
+  Group& operator=(const Group&) = delete;
+
+  // in[i] / out[i]: member i's arrays (inputs are not modified)
+  void execute(void* const* in, void* const* out, int direction, bool use_graph, hipStream_t s);
+  int count() const { return static_cast<int>(members_.size()); }
+  // launches one execute of `direction` issues
+  int num_launches(int direction) const;
+
+ private:
+  struct Member {
+    int rank = 3;
+    bool real = false;
+    Vec gin, gout;
+    std::vector<int> fwd_axes, bwd_axes;  // pass order
+    double scale_f = 1.0, scale_b = 1.0;
+    void* scratch = nullptr;  // complex spectrum for the backward R2C passes
+  };
+  std::vector<LinePass> build_pass(const std::vector<void*>& in, const std::vector<void*>& out, bool forward,
+                                   int k) const;
+  void upload(const std::vector<void*>& in, const std::vector<void*>& out, int dir, hipStream_t s);
+  void launch_all(int dir, hipStream_t s);
+
+  std::vector<Member> members_;
+  bool fp64_ = true;
+  int device_ = 0;
+  int max_passes_ = 0;
+  std::map<int, void*> tw_;
+  // per direction: the pointer set the device tables were built for, the
+  // tables of every pass (LinePass items + virtual-block prefix), members
+  // whose length has no grouped kernel (launched one by one), and a graph
+  struct DirState {
+    std::vector<void*> in, out;
+    bool built = false;
+    std::vector<LinePass*> items;     // device, one array per pass
+    std::vector<int32_t*> first;      // device
+    std::vector<int> count, total, grid;
+    std::vector<size_t> lds;
+    std::vector<std::vector<LinePass>> single;  // host: individually launched passes
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } dir_[2];
+  hipStream_t graph_stream_ = nullptr;
+};
+
+}  // namespace dfft

@@ -28,6 +28,26 @@ Vec to_vec(const size_t* a, std::size_t n) { return Vec(a, a + n); }
 bool contains(const Vec& v, std::size_t x) { return std::find(v.begin(), v.end(), x) != v.end(); }
 }  // namespace
 
+void* upload_twiddles(int n) {
+  const std::size_t es = 16;
+  std::vector<unsigned char> host(es * static_cast<std::size_t>(n));
+  for (int e = 0; e < n; ++e) {
+    const long double a = 6.283185307179586476925286766559005768L * e / n;
+    const long double c = cosl(a), s = -sinl(a);
+    double v[2] = {static_cast<double>(c), static_cast<double>(s)};
+    std::memcpy(&host[es * e], v, es);
+  }
+  void* p = nullptr;
+  hipError_t err = hipMalloc(&p, std::max<std::size_t>(host.size(), 16));
+  if (err != hipSuccess) throw Error(DFFT_ERROR_OUT_OF_MEMORY, "hipMalloc of twiddle table failed");
+  err = hipMemcpy(p, host.data(), host.size(), hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    (void)hipFree(p);
+    throw Error(DFFT_ERROR_HIP, std::string("twiddle upload: ") + hipGetErrorString(err));
+  }
+  return p;
+}
+
 Plan::Plan(const dfft_desc& d) : d_(d) {
   if (d.rank < 2 || d.rank > DFFT_MAX_RANK)
     throw Error(DFFT_ERROR_UNSUPPORTED, "view rank must be in [2, 4]");
@@ -114,20 +134,8 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
   for (std::size_t k = 0; k < axes_.size(); ++k) {
     const int n = static_cast<int>(fft_extents_[k]);
     if (tw_.count(n)) continue;
-    const std::size_t es = 16;
-    std::vector<unsigned char> host(es * n);
-    for (int e = 0; e < n; ++e) {
-      const long double a = 6.283185307179586476925286766559005768L * e / n;
-      const long double c = cosl(a), s = -sinl(a);
-      double v[2] = {static_cast<double>(c), static_cast<double>(s)};
-      std::memcpy(&host[es * e], v, es);
-    }
-    void* p = nullptr;
-    hipError_t err = hipMalloc(&p, std::max<std::size_t>(host.size(), 16));
-    if (err != hipSuccess) throw Error(DFFT_ERROR_OUT_OF_MEMORY, "hipMalloc of twiddle table failed");
-    check_hip(hipMemcpy(p, host.data(), host.size(), hipMemcpyHostToDevice), "twiddle upload");
-    tw_[n] = p;
-    workspace_bytes_ += host.size();
+    tw_[n] = upload_twiddles(n);
+    workspace_bytes_ += 16 * static_cast<std::size_t>(n);
   }
 
   const std::size_t cs = fp64_ ? 16 : 8;

@@ -25,6 +25,10 @@ struct PassSpec {
   int lines_fast = -1;
 };
 
+// device table exp(-2 pi i e / n), e in [0, n): evaluated in long double,
+// rounded once to FP64 (both precisions form butterfly powers from it in FP64)
+void* upload_twiddles(int n);
+
 class Plan {
  public:
   explicit Plan(const dfft_desc& d);

@@ -161,7 +161,8 @@ EXPORTED = [
     "dfft_plan_num_passes",
     "dfft_execute_timed", "dfft_plan_set_option", "dfft_plan_get_option", "dfft_plan_autotune", "dfft_pack",
     "dfft_unpack", "dfft_all2all", "dfft_exchange", "dfft_get_all_slab_topologies",
-    "dfft_get_all_pencil_topologies", "dfft_block_analyses", "dfft_ns_create", "dfft_ns_destroy",
+    "dfft_get_all_pencil_topologies", "dfft_block_analyses", "dfft_group_create", "dfft_group_destroy",
+    "dfft_group_execute", "dfft_group_num_launches", "dfft_ns_create", "dfft_ns_destroy",
     "dfft_ns_local_extents", "dfft_ns_init_taylor_green", "dfft_ns_set_physical", "dfft_ns_set_state",
     "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_energy", "dfft_ns_rhs", "dfft_ns_rhs_tangent",
     "dfft_ns_rhs_adjoint", "dfft_ns_gradient",
@@ -215,6 +216,11 @@ def load_library(path: os.PathLike | None = None):
                                             ip, szp, szp, szp, ip], ctypes.c_int),
         "dfft_block_analyses": ([ctypes.POINTER(_Desc), ctypes.c_int, ctypes.POINTER(BlockInfo), ip, szp],
                                 ctypes.c_int),
+        "dfft_group_create": ([ctypes.c_int, ctypes.POINTER(_Desc), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "dfft_group_destroy": ([ctypes.c_void_p], ctypes.c_int),
+        "dfft_group_execute": ([ctypes.c_void_p, vpp, vpp, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
+                               ctypes.c_int),
+        "dfft_group_num_launches": ([ctypes.c_void_p, ctypes.c_int, ip], ctypes.c_int),
         "dfft_ns_create": ([ctypes.POINTER(NsDesc), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
         "dfft_ns_destroy": ([ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_local_extents": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, szp, szp], ctypes.c_int),
@@ -493,6 +499,56 @@ class Plan:
         if getattr(self, "_plan", None) and self._plan.value:
             self._lib.dfft_plan_destroy(self._plan)
             self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Group:
+    """Independent single-GPU transforms of possibly different extents run
+    together: each line pass of all members is one grouped kernel launch.
+    ``ins[i]`` / ``outs[i]`` are member i's arrays; ``axes`` is one axes list
+    for all members or one per member (default: every axis)."""
+
+    def __init__(self, ins: Sequence, outs: Sequence, axes=None, norm="backward"):
+        import torch
+        lib = load_library()
+        self._lib = lib
+        self._h = ctypes.c_void_p()
+        if len(ins) != len(outs) or not ins:
+            raise DFFTError(ERROR_INVALID_ARGUMENT, "need one output per input")
+        n = len(ins)
+        if axes is None:
+            axes = [list(range(t.dim())) for t in ins]
+        elif not isinstance(axes[0], (list, tuple)):
+            axes = [list(axes)] * n
+        descs = (_Desc * n)()
+        for i, (a, b) in enumerate(zip(ins, outs)):
+            if not b.dtype.is_complex:
+                raise DFFTError(ERROR_INVALID_ARGUMENT, "Group: the output type must be complex")
+            real = not a.dtype.is_complex
+            fp64 = b.dtype == torch.complex128
+            ones = [1] * a.dim()
+            descs[i] = _make_desc(list(a.shape), list(b.shape), axes[i], ones, ones, real, fp64, norm, EXEC_FUSED)
+        _check(lib.dfft_group_create(n, descs, ctypes.byref(self._h)))
+        self.count = n
+
+    def execute(self, ins, outs, direction, use_graph=False, stream=None):
+        _check(self._lib.dfft_group_execute(self._h, _ptrs(ins), _ptrs(outs), int(direction), int(bool(use_graph)),
+                                            _stream_handle(stream)))
+
+    def num_launches(self, direction):
+        v = ctypes.c_int()
+        _check(self._lib.dfft_group_num_launches(self._h, int(direction), ctypes.byref(v)))
+        return int(v.value)
+
+    def destroy(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.dfft_group_destroy(self._h)
+            self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:

@@ -174,6 +174,23 @@ int dfft_exchange(void* const* in, void* const* out, int rank, const size_t* glo
                   const size_t* out_topology, int out_topology_layout, const size_t* out_map, int elem_size,
                   void* stream);
 
+/* ---- groups of independent transforms ----------------------------------- */
+/* A group runs many single-GPU transforms of possibly different extents
+ * (C2C or R2C, one precision, LayoutRight, nranks = 1) together: line pass k
+ * of every member is one grouped kernel launch (members whose line length has
+ * no grouped kernel get launches of their own).  Replaces a loop of
+ * execute(plan_i, in_i, out_i) over separate plans (Plan.hpp:212-238); the
+ * reference batches only equal-size fields, through a leading batch axis
+ * (navier-stokes-MPI-batched/main.cpp:102-107). */
+typedef struct dfft_group dfft_group;
+int dfft_group_create(int count, const dfft_desc* descs, dfft_group** group);
+int dfft_group_destroy(dfft_group* group);
+/* in[i] / out[i]: member i's arrays.  use_graph: the launches are captured
+ * once per direction and pointer set into a hipGraph and replayed after. */
+int dfft_group_execute(dfft_group* group, void* const* in, void* const* out, int direction, int use_graph,
+                       void* stream);
+int dfft_group_num_launches(const dfft_group* group, int direction, int* nlaunches);
+
 /* ---- pseudo-spectral Navier-Stokes (examples/navier-stokes-MPI) -------- */
 /* The example's solver (main.cpp:31-986, utils/math_utils.hpp:47-141) on P =
  * px*py virtual ranks: velocity spectra in the (1,1,px,py) topology of the

@@ -1,0 +1,127 @@
+"""Groups of independent transforms of mixed extents (dfft_group_*): every
+member against the oracle, graph replay against eager, and the 64-member
+mixed batch N in {96, 128, 160}^3 complex64."""
+
+import numpy as np
+import pytest
+
+from helpers import out_extents, rel_l2, seeded_input
+from oracle import fft as F
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(dev, shapes, real, fp64, axes_list, seed):
+    import torch
+    rdt = torch.float64 if fp64 else torch.float32
+    cdt = torch.complex128 if fp64 else torch.complex64
+    xs, ins, outs = [], [], []
+    for i, (shape, axes) in enumerate(zip(shapes, axes_list)):
+        x = seeded_input(shape, real, seed + i)
+        xs.append(x)
+        ins.append(torch.from_numpy(x).to(dev, rdt if real else cdt))
+        outs.append(torch.full(out_extents(shape, axes, real), float("nan"), dtype=cdt, device=dev))
+    return xs, ins, outs
+
+
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("fp64", [True, False])
+def test_mixed_group_matches_oracle(dev, real, fp64):
+    """Lengths with a grouped kernel (16, 32, 64, 96, 160) mixed with lengths
+    launched one by one (24, 12, 7), batched 4-D members, 1- and 2-axis members."""
+    import torch
+    import dfft
+    shapes = [[16, 16, 16], [32, 24, 16], [12, 96, 8], [7, 16, 32], [3, 32, 32, 32], [160, 8, 4], [16, 64],
+              [20, 96]]
+    axes_list = [[0, 1, 2], [0, 1, 2], [0, 1, 2], [0, 1, 2], [1, 2, 3], [0, 2, 1], [0, 1], [1]]
+    xs, ins, outs = _members(dev, shapes, real, fp64, axes_list, 10)
+    g = dfft.Group(ins, outs, axes_list)
+    g.execute(ins, outs, dfft.Direction.forward)
+    backs = [torch.full(list(t.shape), float("nan"), dtype=t.dtype, device=dev) for t in ins]
+    g.execute(outs, backs, dfft.Direction.backward)
+    torch.cuda.synchronize()
+    tol = 1e-12 if fp64 else 2e-6
+    for x, o, b, axes in zip(xs, outs, backs, axes_list):
+        ref = F.fftn(x, axes, real)
+        assert rel_l2(o.cpu().numpy(), ref) <= tol
+        assert rel_l2(b.cpu().numpy(), x) <= tol
+    # the nine members with a grouped length share one launch per pass; 7, 12, 20, 24 get their own
+    assert g.num_launches(dfft.Direction.forward) >= 3
+
+
+def test_group_graph_replay_matches_eager(dev):
+    import torch
+    import dfft
+    shapes = [[32, 32, 32], [16, 96, 16], [160, 16, 8], [24, 16, 16]]
+    axes_list = [[0, 1, 2]] * 4
+    xs, ins, outs = _members(dev, shapes, False, False, axes_list, 20)
+    eager = [torch.empty_like(t) for t in outs]
+    g = dfft.Group(ins, outs, axes_list, norm="ortho")
+    g.execute(ins, eager, dfft.Direction.forward)
+    for rep in range(4):
+        for t in outs:
+            t.fill_(float("nan"))
+        g.execute(ins, outs, dfft.Direction.forward, use_graph=True)
+        torch.cuda.synchronize()
+        for a, b in zip(outs, eager):
+            assert bool((a == b).all()), rep
+    # new pointers rebuild the tables and the graph
+    outs2 = [torch.empty_like(t) for t in outs]
+    g.execute(ins, outs2, dfft.Direction.forward, use_graph=True)
+    g.execute(ins, outs2, dfft.Direction.forward, use_graph=True)
+    torch.cuda.synchronize()
+    for a, b in zip(outs2, eager):
+        assert bool((a == b).all())
+
+
+def test_group_rejects_bad_members(dev):
+    import torch
+    import dfft
+    a = torch.zeros(8, 8, 8, dtype=torch.complex64, device=dev)
+    b = torch.zeros(8, 8, 8, dtype=torch.complex128, device=dev)
+    with pytest.raises(dfft.DFFTError) as e:
+        dfft.Group([a, b], [a.clone(), b.clone()])
+    assert e.value.code == dfft.ERROR_MISMATCH
+    r = torch.zeros(8, 8, 8, dtype=torch.float32, device=dev)
+    c = torch.zeros(8, 8, 5, dtype=torch.complex64, device=dev)
+    g = dfft.Group([r], [c])
+    with pytest.raises(dfft.DFFTError) as e:
+        g.execute([c], [c], dfft.Direction.forward)
+    assert e.value.code == dfft.ERROR_ALIAS
+    with pytest.raises(dfft.DFFTError) as e:
+        dfft.Group([r], [torch.zeros(8, 8, 6, dtype=torch.complex64, device=dev)])
+    assert e.value.code == dfft.ERROR_INVALID_EXTENTS
+
+
+def test_mixed_batch_64_transforms_96_128_160(dev):
+    """64 C2C transforms, N cycling through {96, 128, 160}^3, complex64, uniform [-1, 1]:
+    one launch per pass; spot members against the FP64 oracle, all round trips."""
+    import torch
+    import dfft
+    sizes = [(96, 128, 160)[i % 3] for i in range(64)]
+    gen = torch.Generator(device=dev).manual_seed(1234)
+    ins, outs = [], []
+    for n in sizes:
+        re = torch.rand((n, n, n), generator=gen, device=dev, dtype=torch.float32) * 2 - 1
+        im = torch.rand((n, n, n), generator=gen, device=dev, dtype=torch.float32) * 2 - 1
+        ins.append(torch.complex(re, im))
+        outs.append(torch.empty((n, n, n), dtype=torch.complex64, device=dev))
+    g = dfft.Group(ins, outs)
+    assert g.num_launches(dfft.Direction.forward) == 3
+    assert g.num_launches(dfft.Direction.backward) == 3
+    g.execute(ins, outs, dfft.Direction.forward, use_graph=True)
+    backs = [torch.empty_like(t) for t in ins]
+    g.execute(outs, backs, dfft.Direction.backward, use_graph=True)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, (x, b) in enumerate(zip(ins, backs)):
+        num = float(torch.linalg.vector_norm(b - x))
+        den = float(torch.linalg.vector_norm(x))
+        worst = max(worst, num / den)
+    assert worst <= 1e-6
+    for i in (0, 1, 2, 63):
+        x = ins[i].cpu().numpy().astype(np.complex128)
+        ref = np.fft.fftn(x)
+        err = rel_l2(outs[i].cpu().numpy().astype(np.complex128), ref)
+        print(f"member {i} N={sizes[i]} rel-L2 vs FP64 {err:.2e}")
+        assert err <= 1e-6
