@@ -283,6 +283,13 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 8, "launches": 3}))
         return
+    sections = set(args.sections.split(","))
+    if "fft" not in sections:
+        # profiling runs of single sections: no headline line
+        line = {"sections": sorted(sections)}
+        run_sections(torch, dfft, dev, args, sections, line)
+        print(json.dumps(line), flush=True)
+        return
     results = {}
     precisions = (("fp64", torch.complex128), ("fp32", torch.complex64))
     if args.only:
@@ -342,7 +349,14 @@ def main():
                     "fp32": {"fwd": results["fp32"]["fwd_pass_ms"], "bwd": results["fp32"]["bwd_pass_ms"]}},
         "roofline": roofline,
     }
-    sections = set(args.sections.split(","))
+    run_sections(torch, dfft, dev, args, sections, line)
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle) ...")
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+def run_sections(torch, dfft, dev, args, sections, line):
     if "ns" in sections:
         log("[bench] NS Taylor-Green 256^3 ...")
         line["ns"] = bench_ns(torch, max(3, args.steps))
@@ -359,10 +373,6 @@ def main():
         log("[bench] precision FP32 vs FP64 ...")
         line["precision"] = bench_precision(torch, dfft, dev)
         torch.cuda.empty_cache()
-    if not args.no_cpu_baseline:
-        log("[bench] cpu baseline (oracle) ...")
-        line["cpu_baseline"] = cpu_baseline()
-    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

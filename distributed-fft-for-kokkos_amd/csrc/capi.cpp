@@ -478,6 +478,35 @@ int dfft_ns_energy(dfft_ns* ns, double* energy, void* stream) {
   });
 }
 
+int dfft_ns_velocity(dfft_ns* ns, void* const* u3, void* stream) {
+  return guarded([&] {
+    need(ns && u3, "null pointer");
+    ns->impl->velocity(u3, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_vorticity(dfft_ns* ns, void* const* w3, void* stream) {
+  return guarded([&] {
+    need(ns && w3, "null pointer");
+    ns->impl->vorticity(w3, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_energy_spectrum(dfft_ns* ns, int num_bins, double* e_spec, double* k_vals, void* stream) {
+  return guarded([&] {
+    need(ns && e_spec, "null pointer");
+    ns->impl->energy_spectrum(num_bins, e_spec, k_vals, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfft_ns_diag(dfft_ns* ns, const char* out_dir, int diag_it, double* energy, void* stream) {
+  return guarded([&] {
+    need(ns && out_dir, "null pointer");
+    const double e = ns->impl->diag(out_dir, diag_it, static_cast<hipStream_t>(stream));
+    if (energy) *energy = e;
+  });
+}
+
 int dfft_ns_rhs(dfft_ns* ns, void* const* y3, void* const* out3, void* stream) {
   return guarded([&] {
     need(ns && y3 && out3, "null pointer");

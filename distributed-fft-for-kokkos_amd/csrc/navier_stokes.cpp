@@ -87,6 +87,8 @@ NavierStokes::~NavierStokes() {
   if (graph_stream_) (void)hipStreamDestroy(graph_stream_);
   for (void* p : all_) (void)hipFree(p);
   if (partials_) (void)hipFree(partials_);
+  if (spec_partials_) (void)hipFree(spec_partials_);
+  if (spec_bins_) (void)hipFree(spec_bins_);
   if (esum_) (void)hipFree(esum_);
 }
 

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "ns_kernels.hpp"
@@ -37,6 +38,15 @@ class NavierStokes {
   const NsBlock& spectral_block(int r) const { return sblk_.at(r); }
   const NsBlock& physical_block(int r) const { return pblk_.at(r); }
   int nranks() const { return P_; }
+
+  // diagnostics (ns_diag.cpp; main.cpp:812-935)
+  void velocity(void* const* u3, hipStream_t s);   // physical velocity of the state
+  void vorticity(void* const* w3, hipStream_t s);  // physical vorticity
+  // radial spectrum of 0.5 f |u_k|^2 over num_bins bins of [0, k_max]; k_vals = bin midpoints
+  void energy_spectrum(int num_bins, double* e_spec, double* k_vals, hipStream_t s);
+  // the reference's diag(): per-rank u, v, w, vor_x, vor_y, vor_z and the
+  // E_spec / k_vals files under <out_dir>_px<px>_py<py>; returns the energy
+  double diag(const std::string& out_dir, int diag_it, hipStream_t s);
 
   // reverse mode (ns_adjoint.cpp)
   double gradient(int nsteps, int checkpoint_every, void* const* u0, void* const* grad, hipStream_t s);
@@ -65,6 +75,8 @@ class NavierStokes {
   std::vector<void*> aux_s3a_, aux_s3b_, aux_p3_;  // tangent/adjoint work fields (allocated on first use)
   std::vector<void*> all_;
   double* partials_ = nullptr;
+  double* spec_partials_ = nullptr;  // diagnostics: per-block spectrum rows (allocated on first use)
+  double* spec_bins_ = nullptr;
   double* esum_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   hipGraph_t graph_ = nullptr;

@@ -450,3 +450,81 @@ hipError_t ns_scale(int64_t n, double a, double* x, hipStream_t s) {
 }
 
 }  // namespace dfft
+
+namespace dfft {
+
+// ---------------------------------------------------------------------------
+// Diagnostics (main.cpp:781-910): vorticity and the radial energy spectrum.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) vorticity_kernel(const NsGrid g, const NsBlock b, const double2* y,
+                                                        double2* w) {
+  const int64_t m_tot = static_cast<int64_t>(b.ext[0]) * b.ext[1] * b.ext[2];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; m < m_tot; m += stride) {
+    const Mode md = mode_of(g, b, m);
+    const double2 u = y[m], v = y[m_tot + m], z = y[2 * m_tot + m];
+    // i (ky w - kz v), i (kz u - kx w), i (kx v - ky u)
+    w[m] = ci(double2{md.ky * z.x - md.kz * v.x, md.ky * z.y - md.kz * v.y});
+    w[m_tot + m] = ci(double2{md.kz * u.x - md.kx * z.x, md.kz * u.y - md.kx * z.y});
+    w[2 * m_tot + m] = ci(double2{md.kx * v.x - md.ky * u.x, md.kx * v.y - md.ky * u.y});
+  }
+}
+
+hipError_t ns_vorticity(const NsGrid& g, const NsBlock& b, const double2* y3, double2* w3, hipStream_t s) {
+  const int64_t m = static_cast<int64_t>(b.ext[0]) * b.ext[1] * b.ext[2];
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(vorticity_kernel, dim3(blocks_for(m)), dim3(256), 0, s, g, b, y3, w3);
+  return hipGetLastError();
+}
+
+// Each block bins its points into LDS (bins [k_i, k_{i+1}), the reference's
+// half-open test) and writes one row of per-bin partial sums.
+__global__ void __launch_bounds__(256) spectrum_kernel(const NsGrid g, const NsBlock b, const double2* y,
+                                                       const double* bins, int nb, double* partials) {
+  __shared__ double acc[kMaxSpectrumBins];
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc[i] = 0.0;
+  __syncthreads();
+  const int64_t m_tot = static_cast<int64_t>(b.ext[0]) * b.ext[1] * b.ext[2];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const double width = bins[nb] / nb;
+  for (int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; m < m_tot; m += stride) {
+    const Mode md = mode_of(g, b, m);
+    const double k = sqrt(md.ksq);
+    int i = width > 0.0 ? static_cast<int>(k / width) : 0;
+    i = i < 0 ? 0 : (i > nb - 1 ? nb - 1 : i);
+    while (i > 0 && k < bins[i]) --i;
+    while (i < nb - 1 && k >= bins[i + 1]) ++i;
+    if (!(k >= bins[i] && k < bins[i + 1])) continue;
+    double e = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const double2 v = y[c * m_tot + m];
+      e += v.x * v.x + v.y * v.y;
+    }
+    atomicAdd(&acc[i], 0.5 * (md.kz0 ? 1.0 : 2.0) * e);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) partials[static_cast<int64_t>(blockIdx.x) * nb + i] = acc[i];
+}
+
+hipError_t ns_energy_spectrum_partials(const NsGrid& g, const NsBlock& b, const double2* y3, const double* bins,
+                                       int nb, double* partials, int nblocks, hipStream_t s) {
+  if (nb < 1 || nb > kMaxSpectrumBins) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(spectrum_kernel, dim3(nblocks), dim3(256), 0, s, g, b, y3, bins, nb, partials);
+  return hipGetLastError();
+}
+
+// out[i] = sum over rows r of partials[r * nb + i], rows in order
+__global__ void __launch_bounds__(256) bin_sum_kernel(const double* partials, int rows, int nb, double* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
+    double a = 0.0;
+    for (int r = 0; r < rows; ++r) a += partials[static_cast<int64_t>(r) * nb + i];
+    out[i] = a;
+  }
+}
+
+hipError_t ns_bin_sum(const double* partials, int rows, int nb, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(bin_sum_kernel, dim3(1), dim3(256), 0, s, partials, rows, nb, out);
+  return hipGetLastError();
+}
+
+}  // namespace dfft

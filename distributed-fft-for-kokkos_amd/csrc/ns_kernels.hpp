@@ -82,4 +82,14 @@ hipError_t ns_dealias(const NsGrid& g, const NsBlock& b, const double2* src3, do
 // x *= a over n doubles
 hipError_t ns_scale(int64_t n, double a, double* x, hipStream_t s);
 
+// diagnostics (main.cpp:781-910)
+constexpr int kMaxSpectrumBins = 256;
+// w = curl of y: i (ky w - kz v), i (kz u - kx w), i (kx v - ky u)
+hipError_t ns_vorticity(const NsGrid& g, const NsBlock& b, const double2* y3, double2* w3, hipStream_t s);
+// per-block partial sums of 0.5 f (|u|^2 + |v|^2 + |w|^2) over the half-open
+// radial bins [bins[i], bins[i+1]), f = 1 on the global kz = 0 plane, 2 elsewhere
+hipError_t ns_energy_spectrum_partials(const NsGrid& g, const NsBlock& b, const double2* y3, const double* bins,
+                                       int nb, double* partials, int nblocks, hipStream_t s);
+hipError_t ns_bin_sum(const double* partials, int rows, int nb, double* out, hipStream_t s);
+
 }  // namespace dfft

@@ -164,7 +164,8 @@ EXPORTED = [
     "dfft_get_all_pencil_topologies", "dfft_block_analyses", "dfft_group_create", "dfft_group_destroy",
     "dfft_group_execute", "dfft_group_num_launches", "dfft_ns_create", "dfft_ns_destroy",
     "dfft_ns_local_extents", "dfft_ns_init_taylor_green", "dfft_ns_set_physical", "dfft_ns_set_state",
-    "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_energy", "dfft_ns_rhs", "dfft_ns_rhs_tangent",
+    "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_energy", "dfft_ns_velocity", "dfft_ns_vorticity",
+    "dfft_ns_energy_spectrum", "dfft_ns_diag", "dfft_ns_rhs", "dfft_ns_rhs_tangent",
     "dfft_ns_rhs_adjoint", "dfft_ns_gradient",
 ]
 
@@ -230,6 +231,12 @@ def load_library(path: os.PathLike | None = None):
         "dfft_ns_get_state": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_step": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_energy": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_velocity": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_vorticity": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_energy_spectrum": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double), ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_diag": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                          ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_rhs": ([ctypes.c_void_p, vpp, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_rhs_tangent": ([ctypes.c_void_p, vpp, vpp, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_rhs_adjoint": ([ctypes.c_void_p, vpp, vpp, vpp, ctypes.c_void_p], ctypes.c_int),

@@ -90,6 +90,31 @@ class NavierStokes:
         _check(self._lib.dfft_ns_energy(self._h, ctypes.byref(e), _stream_handle(stream)))
         return float(e.value)
 
+    # -- diagnostics (main.cpp:812-935) --------------------------------------
+    def velocity(self, stream=None):
+        u = self.physical_fields()
+        _check(self._lib.dfft_ns_velocity(self._h, _ptrs(u), _stream_handle(stream)))
+        return u
+
+    def vorticity(self, stream=None):
+        w = self.physical_fields()
+        _check(self._lib.dfft_ns_vorticity(self._h, _ptrs(w), _stream_handle(stream)))
+        return w
+
+    def energy_spectrum(self, num_bins=50, stream=None):
+        """(E_spec, k_vals): radial bins of 0.5 f |u_k|^2 (unscaled, as the reference's E_spec)."""
+        e = (ctypes.c_double * num_bins)()
+        k = (ctypes.c_double * num_bins)()
+        _check(self._lib.dfft_ns_energy_spectrum(self._h, int(num_bins), e, k, _stream_handle(stream)))
+        return list(e), list(k)
+
+    def diag(self, out_dir, diag_it, stream=None) -> float:
+        """Writes the reference's per-rank .dat files; returns the kinetic energy."""
+        e = ctypes.c_double()
+        _check(self._lib.dfft_ns_diag(self._h, str(out_dir).encode(), int(diag_it), ctypes.byref(e),
+                                      _stream_handle(stream)))
+        return float(e.value)
+
     # -- derivatives (divergence form) --------------------------------------
     def rhs(self, y, stream=None):
         self._check_set(y, 1)

@@ -221,6 +221,18 @@ int dfft_ns_get_state(dfft_ns* ns, void* const* y3, void* stream);
 int dfft_ns_step(dfft_ns* ns, int nsteps, int use_graph, void* stream);
 /* kinetic energy, FP64 reduction (diag_energy, main.cpp:855-910) */
 int dfft_ns_energy(dfft_ns* ns, double* energy, void* stream);
+/* diagnostics (diag_fields / diag_energy, main.cpp:812-935): physical
+ * velocity and vorticity of the state into P blocks of (3, ...) float64; the
+ * radial spectrum sum 0.5 f |u_k|^2 over num_bins half-open bins of
+ * [0, k_max] (f from the global kz index: 1 on kz = 0, 2 elsewhere) with the
+ * bin midpoints; and the reference's raw files: per rank r with (rx, ry) =
+ * rank_to_coord({px, py}, r), <out_dir>_px<px>_py<py>/<label>_rx###_ry###_
+ * ##########.dat for u, v, w, vor_x, vor_y, vor_z (LayoutRight float64
+ * blocks), plus E_spec and k_vals (50 bins) under rx000_ry000. */
+int dfft_ns_velocity(dfft_ns* ns, void* const* u3, void* stream);
+int dfft_ns_vorticity(dfft_ns* ns, void* const* w3, void* stream);
+int dfft_ns_energy_spectrum(dfft_ns* ns, int num_bins, double* e_spec, double* k_vals, void* stream);
+int dfft_ns_diag(dfft_ns* ns, const char* out_dir, int diag_it, double* energy, void* stream);
 /* right-hand side R(y), its tangent R'(y)d and adjoint R'(y)^*l (form 0);
  * no counterpart in the reference -- for gradient-based callers */
 int dfft_ns_rhs(dfft_ns* ns, void* const* y3, void* const* out3, void* stream);

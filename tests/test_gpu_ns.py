@@ -205,3 +205,56 @@ def test_autograd_harness(dev):
     assert abs(float(e) - j) == 0.0
     for a, b in zip(gr, grad):
         assert torch.allclose(a, 3.0 * b, rtol=0, atol=0)
+
+
+def test_diagnostics_of_taylor_green(dev, tmp_path):
+    """diag_fields / diag_energy (main.cpp:812-910): per-rank u, v, w and vorticity files
+    named as the reference names them, against the Taylor-Green closed forms; the
+    spectrum holds all energy in the |k| = sqrt(3) bin."""
+    n, px, py = 16, 2, 2
+    s = _solver(n, px, py, 0)
+    s.init_taylor_green(1.0)
+    e = s.diag(str(tmp_path / "data"), 7)
+    assert abs(e - np.pi ** 3) < 1e-12 * np.pi ** 3
+    g = NS.Grid(n)
+    X, Y, Z = np.meshgrid(g.x, g.x, g.x, indexing="ij")
+    want = {"u": np.sin(X) * np.cos(Y) * np.cos(Z), "v": -np.cos(X) * np.sin(Y) * np.cos(Z), "w": 0 * X,
+            "vor_x": -np.cos(X) * np.sin(Y) * np.sin(Z), "vor_y": -np.sin(X) * np.cos(Y) * np.sin(Z),
+            "vor_z": 2 * np.sin(X) * np.sin(Y) * np.cos(Z)}
+    d = tmp_path / f"data_px{px}_py{py}"
+    for r in range(px * py):
+        ext, st = s.local_extents(r, 0)
+        rx, ry = r // py, r % py
+        sl = tuple(slice(a, a + b) for a, b in zip(st, ext))
+        for label, ref in want.items():
+            f = d / f"{label}_rx{rx:03d}_ry{ry:03d}_{7:010d}.dat"
+            got = np.fromfile(f, dtype=np.float64).reshape(ext)
+            assert np.abs(got - ref[sl]).max() < 1e-13, (label, r)
+    e_spec = np.fromfile(d / f"E_spec_rx000_ry000_{7:010d}.dat", dtype=np.float64)
+    k_vals = np.fromfile(d / f"k_vals_rx000_ry000_{7:010d}.dat", dtype=np.float64)
+    assert e_spec.shape == (50,) and k_vals.shape == (50,)
+    k_max = np.sqrt(3.0) * (n // 2)
+    bins = np.linspace(0.0, k_max, 51)
+    assert np.allclose(k_vals, 0.5 * (bins[:-1] + bins[1:]), rtol=0, atol=1e-14)
+    hit = int(np.sqrt(3.0) // (k_max / 50))
+    coef = (2 * np.pi) ** 3 / n ** 6
+    assert abs(e_spec[hit] * coef - np.pi ** 3) < 1e-12 * np.pi ** 3
+    assert np.abs(np.delete(e_spec, hit)).max() == 0.0
+
+
+def test_energy_spectrum_sums_to_energy(dev):
+    n = 24
+    s = _solver(n, 2, 3, 0)
+    y, = _random_states(s, dev, [9])
+    s.set_state(y)
+    e_spec, k_vals = s.energy_spectrum(50)
+    coef = (2 * np.pi) ** 3 / n ** 6
+    assert abs(sum(e_spec) * coef - s.energy()) < 1e-12 * s.energy()
+    # velocity and vorticity agree with the oracle's backward transforms of the state
+    ys = _gather(s, s.get_state(), 1)
+    g = NS.Grid(n)
+    u_ref = np.stack([NS.backward(c, n) for c in ys])
+    assert _rel(_gather(s, s.velocity(), 0), u_ref) < 1e-12
+    w_hat = [1j * (g.KY * ys[2] - g.KZ * ys[1]), 1j * (g.KZ * ys[0] - g.KX * ys[2]),
+             1j * (g.KX * ys[1] - g.KY * ys[0])]
+    assert _rel(_gather(s, s.vorticity(), 0), np.stack([NS.backward(c, n) for c in w_hat])) < 1e-12
