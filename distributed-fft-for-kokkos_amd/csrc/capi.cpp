@@ -401,6 +401,27 @@ int dfft_group_num_launches(const dfft_group* group, int direction, int* nlaunch
   });
 }
 
+int dfft_group_set_option(dfft_group* group, const char* key, long value) {
+  return guarded([&] {
+    need(group && key, "null pointer");
+    group->impl->set_option(key, value);
+  });
+}
+
+int dfft_group_get_option(const dfft_group* group, const char* key, long* value) {
+  return guarded([&] {
+    need(group && key && value, "null pointer");
+    *value = group->impl->get_option(key);
+  });
+}
+
+int dfft_group_autotune(dfft_group* group, void* const* in, void* const* out, void* stream) {
+  return guarded([&] {
+    need(group && in && out, "null pointer");
+    group->impl->autotune(in, out, static_cast<hipStream_t>(stream));
+  });
+}
+
 int dfft_ns_create(const dfft_ns_desc* d, dfft_ns** ns) {
   return guarded([&] {
     need(d && ns, "null pointer");

@@ -140,6 +140,27 @@ template <int r, typename C> __device__ __forceinline__ void bfly(C* a) {
 // ---------------------------------------------------------------------------
 // Element access of a line.
 // ---------------------------------------------------------------------------
+// Global element access: nontemporal (streaming) or cached.
+template <typename T> __device__ __forceinline__ T gload(const T* p, bool nt) {
+  return nt ? __builtin_nontemporal_load(p) : *p;
+}
+template <typename T> __device__ __forceinline__ void gstore(T* p, T v, bool nt) {
+  if (nt) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+template <typename Real> __device__ __forceinline__ cplx_t<Real> gload_c(const cplx_t<Real>* p, bool nt) {
+  typedef Real V2 __attribute__((ext_vector_type(2)));
+  const V2 t = gload(reinterpret_cast<const V2*>(p), nt);
+  return cplx_t<Real>{t.x, t.y};
+}
+template <typename Real> __device__ __forceinline__ void gstore_c(cplx_t<Real>* p, cplx_t<Real> v, bool nt) {
+  typedef Real V2 __attribute__((ext_vector_type(2)));
+  V2 t;
+  t.x = v.x;
+  t.y = v.y;
+  gstore(reinterpret_cast<V2*>(p), t, nt);
+}
+
 template <typename Real> struct LineIO {
   using C = cplx_t<Real>;
   const char* ib;
@@ -148,19 +169,20 @@ template <typename Real> struct LineIO {
   int32_t kind, n, inverse;
   Real scale;
   Real hw;  // interior half-spectrum weight (1: none)
+  bool nt;  // nontemporal global accesses
 
   __device__ __forceinline__ C load(int32_t pos) const {
     C v;
     if (kind == kC2C) {
-      v = reinterpret_cast<const C*>(ib)[pos * is];
+      v = gload_c<Real>(reinterpret_cast<const C*>(ib) + pos * is, nt);
     } else if (kind == kR2C) {
-      v = C{reinterpret_cast<const Real*>(ib)[pos * is], Real(0)};
+      v = C{gload(reinterpret_cast<const Real*>(ib) + pos * is, nt), Real(0)};
     } else {
       const int32_t nh = n / 2 + 1;
       if (pos < nh) {
-        v = reinterpret_cast<const C*>(ib)[pos * is];
+        v = gload_c<Real>(reinterpret_cast<const C*>(ib) + pos * is, nt);
       } else {
-        v = cconj(reinterpret_cast<const C*>(ib)[(n - pos) * is]);
+        v = cconj(gload_c<Real>(reinterpret_cast<const C*>(ib) + (n - pos) * is, nt));
       }
       if (pos == 0 || 2 * pos == n) {
         v.y = Real(0);
@@ -176,15 +198,15 @@ template <typename Real> struct LineIO {
     v.x *= scale;
     v.y *= scale;
     if (kind == kC2C) {
-      reinterpret_cast<C*>(ob)[pos * os] = v;
+      gstore_c<Real>(reinterpret_cast<C*>(ob) + pos * os, v, nt);
     } else if (kind == kR2C) {
       if (pos != 0 && 2 * pos != n) {
         v.x *= hw;
         v.y *= hw;
       }
-      if (2 * pos <= n) reinterpret_cast<C*>(ob)[pos * os] = v;
+      if (2 * pos <= n) gstore_c<Real>(reinterpret_cast<C*>(ob) + pos * os, v, nt);
     } else {
-      reinterpret_cast<Real*>(ob)[pos * os] = v.x;
+      gstore(reinterpret_cast<Real*>(ob) + pos * os, v.x, nt);
     }
   }
 };
@@ -219,6 +241,7 @@ __device__ __forceinline__ LineIO<Real> make_line_io(const LinePass& p, int32_t 
   io.inverse = p.inverse;
   io.scale = static_cast<Real>(p.scale);
   io.hw = p.herm_weight != 0.0 ? static_cast<Real>(p.herm_weight) : Real(1);
+  io.nt = p.nontemporal != 0;
   return io;
 }
 
@@ -406,9 +429,15 @@ __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* 
                                                                 int32_t count, int32_t total_blocks) {
   extern __shared__ unsigned char smem_raw[];
   cplx_t<Real>* lds = reinterpret_cast<cplx_t<Real>*>(smem_raw);
-  int32_t i = 0;
   for (int32_t vb = blockIdx.x; vb < total_blocks; vb += gridDim.x) {
-    while (i + 1 < count && first_block[i + 1] <= vb) ++i;  // vb only grows: a forward scan
+    // member of virtual block vb: the last i with first_block[i] <= vb
+    int32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (first_block[mid] <= vb) lo = mid;
+      else hi = mid - 1;
+    }
+    const int32_t i = lo;
     const LinePass& p = items[i];
     const int32_t blk = vb - first_block[i];
     switch (p.n) {
@@ -461,6 +490,7 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
       io.inverse = p.inverse;
       io.scale = static_cast<Real>(p.scale);
       io.hw = p.herm_weight != 0.0 ? static_cast<Real>(p.herm_weight) : Real(1);
+      io.nt = false;
       if (p.kind == kC2R && pos >= n / 2 + 1) {
         idx[p.axis] = n - pos;
       } else {
@@ -532,6 +562,7 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
     io.inverse = p.inverse;
     io.scale = static_cast<Real>(p.scale);
     io.hw = Real(1);
+    io.nt = false;
     io.ob = element_ptr(p.out, idx, oes, p.axis, st);
     io.os = 0;
     // store(0, .) sees position 0, so the interior weight is applied here
@@ -650,33 +681,35 @@ hipError_t dispatch(const LinePass& p, hipStream_t s) {
 
 namespace {
 // lines per block: as many as the block's lanes hold, capped so the lines
-// plus the spare slot fit the 64 KB default dynamic LDS
+// plus the spare slot fit `cap` bytes of dynamic LDS (at most 64 KB)
 template <typename Real, int N>
-void group_shape_of(int* B, size_t* lds) {
+void group_shape_of(size_t cap, int* B, size_t* lds) {
   using Cfg = FastCfg<Real, N, 8>;
   const size_t line_bytes = static_cast<size_t>(Cfg::LDSN) * sizeof(cplx_t<Real>);
+  if (cap > kGroupLds) cap = kGroupLds;
   int b = kGroupThreads / Cfg::T;
-  while (b > 1 && static_cast<size_t>(b + 1) * line_bytes > kGroupLds) --b;
+  while (b > 1 && static_cast<size_t>(b + 1) * line_bytes > cap) --b;
   *B = b;
   *lds = static_cast<size_t>(b + 1) * line_bytes;
 }
 template <typename Real>
-bool group_shape(int n, int* B, size_t* lds) {
+bool group_shape(int n, size_t cap, int* B, size_t* lds) {
   switch (n) {
-    case 16: group_shape_of<Real, 16>(B, lds); return true;
-    case 32: group_shape_of<Real, 32>(B, lds); return true;
-    case 64: group_shape_of<Real, 64>(B, lds); return true;
-    case 96: group_shape_of<Real, 96>(B, lds); return true;
-    case 128: group_shape_of<Real, 128>(B, lds); return true;
-    case 160: group_shape_of<Real, 160>(B, lds); return true;
-    case 256: group_shape_of<Real, 256>(B, lds); return true;
+    case 16: group_shape_of<Real, 16>(cap, B, lds); return true;
+    case 32: group_shape_of<Real, 32>(cap, B, lds); return true;
+    case 64: group_shape_of<Real, 64>(cap, B, lds); return true;
+    case 96: group_shape_of<Real, 96>(cap, B, lds); return true;
+    case 128: group_shape_of<Real, 128>(cap, B, lds); return true;
+    case 160: group_shape_of<Real, 160>(cap, B, lds); return true;
+    case 256: group_shape_of<Real, 256>(cap, B, lds); return true;
     default: return false;
   }
 }
 }  // namespace
 
-bool group_line_shape(int n, bool fp64, int* lines_per_block, size_t* lds_bytes) {
-  return fp64 ? group_shape<double>(n, lines_per_block, lds_bytes) : group_shape<float>(n, lines_per_block, lds_bytes);
+bool group_line_shape(int n, bool fp64, size_t lds_cap, int* lines_per_block, size_t* lds_bytes) {
+  return fp64 ? group_shape<double>(n, lds_cap, lines_per_block, lds_bytes)
+              : group_shape<float>(n, lds_cap, lines_per_block, lds_bytes);
 }
 
 int group_grid(size_t lds_bytes, bool fp64) {

@@ -186,7 +186,7 @@ std::vector<LinePass> Group::build_pass(const std::vector<void*>& in, const std:
     int B = 0;
     std::size_t lds = 0;
     p.values_per_thread = 8;
-    p.lines_per_block = group_line_shape(p.n, fp64_, &B, &lds) ? B : 0;
+    p.lines_per_block = group_line_shape(p.n, fp64_, lds_cap_, &B, &lds) ? B : 0;
     passes.push_back(p);
   }
   return passes;
@@ -205,6 +205,14 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
   ds.single.assign(np, {});
   for (std::size_t k = 0; k < np; ++k) {
     std::vector<LinePass> all = build_pass(in, out, dir == 0, static_cast<int>(k));
+    double bytes = 0.0;  // array bytes the pass streams through, all members
+    for (const LinePass& p : all) {
+      double e = 1.0;
+      for (int a = 0; a < p.in.ndim; ++a) e *= static_cast<double>(p.in.ext[a]);
+      bytes += e * (p.kind == kR2C ? 0.5 : 1.0) * (fp64_ ? 16.0 : 8.0);
+    }
+    const int nt = nt_ >= 0 ? nt_ : (bytes >= 256.0 * 1024 * 1024 ? 1 : 0);
+    for (LinePass& p : all) p.nontemporal = nt;
     std::vector<LinePass> grouped;
     std::vector<int32_t> first{0};
     std::size_t lds = 0;
@@ -212,7 +220,7 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
       int B = 0;
       std::size_t l = 0;
       if (p.nlines == 0) continue;
-      if (group_line_shape(p.n, fp64_, &B, &l)) {
+      if (group_line_shape(p.n, fp64_, lds_cap_, &B, &l)) {
         grouped.push_back(p);
         const int64_t nb = first.back() + (static_cast<int64_t>(p.nlines) + B - 1) / B;
         if (nb >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many blocks in one grouped pass");
@@ -229,8 +237,9 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
     ds.total[k] = first.back();
     ds.lds[k] = lds;
     if (grouped.empty()) continue;
-    ds.grid[k] = group_grid(lds, fp64_);
-    if (ds.grid[k] <= 0) throw Error(DFFT_ERROR_HIP, "occupancy query of the grouped kernel failed");
+    const int resident = group_grid(lds, fp64_);
+    if (resident <= 0) throw Error(DFFT_ERROR_HIP, "occupancy query of the grouped kernel failed");
+    ds.grid[k] = grid_mult_ > 0 ? resident * grid_mult_ : first.back();
     if (!ds.items[k]) {
       check(hipMalloc(&ds.items[k], sizeof(LinePass) * members_.size()), "hipMalloc");
       check(hipMalloc(&ds.first[k], sizeof(int32_t) * (members_.size() + 1)), "hipMalloc");
@@ -264,12 +273,97 @@ int Group::num_launches(int direction) const {
       const std::vector<int>& order = direction == DFFT_FORWARD ? m.fwd_axes : m.bwd_axes;
       int B = 0;
       std::size_t l = 0;
-      if (group_line_shape(static_cast<int>(m.gin[order[k]]), fp64_, &B, &l)) grouped = true;
+      if (group_line_shape(static_cast<int>(m.gin[order[k]]), fp64_, lds_cap_, &B, &l)) grouped = true;
       else ++n;
     }
     if (grouped) ++n;
   }
   return n;
+}
+
+void Group::invalidate() {
+  for (auto& ds : dir_) {
+    if (ds.exec) {
+      if (graph_stream_) (void)hipStreamSynchronize(graph_stream_);
+      (void)hipGraphExecDestroy(ds.exec);
+      ds.exec = nullptr;
+    }
+    if (ds.graph) {
+      (void)hipGraphDestroy(ds.graph);
+      ds.graph = nullptr;
+    }
+    ds.built = false;
+  }
+}
+
+void Group::set_option(const std::string& key, long value) {
+  if (key == "lds_cap") {
+    if (value < 1024 || value > static_cast<long>(kGroupLds))
+      throw Error(DFFT_ERROR_INVALID_ARGUMENT, "lds_cap must be in [1024, 65536]");
+    lds_cap_ = static_cast<std::size_t>(value);
+  } else if (key == "nt") {
+    if (value < -1 || value > 1) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "nt must be -1, 0 or 1");
+    nt_ = static_cast<int>(value);
+  } else if (key == "grid_mult") {
+    if (value < 0 || value > 64) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "grid_mult must be in [0, 64]");
+    grid_mult_ = static_cast<int>(value);
+  } else {
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown group option: " + key);
+  }
+  invalidate();
+}
+
+long Group::get_option(const std::string& key) const {
+  if (key == "lds_cap") return static_cast<long>(lds_cap_);
+  if (key == "grid_mult") return grid_mult_;
+  if (key == "nt") return nt_;
+  throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown group option: " + key);
+}
+
+void Group::autotune(void* const* in, void* const* out, hipStream_t s) {
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "hipEventCreate");
+  check(hipEventCreate(&e1), "hipEventCreate");
+  float best = -1.0f;
+  std::size_t best_cap = lds_cap_;
+  int best_mult = grid_mult_, best_nt = nt_;
+  try {
+    for (std::size_t cap : {std::size_t(65536), std::size_t(49152), std::size_t(32768), std::size_t(24576)}) {
+      for (int multnt = 0; multnt < 8; ++multnt) {
+        const int mult = (multnt & 3) == 3 ? 0 : (1 << (multnt & 3));
+        const int nt = multnt >> 2;
+        set_option("lds_cap", static_cast<long>(cap));
+        set_option("grid_mult", mult);
+        set_option("nt", nt);
+        execute(in, out, DFFT_FORWARD, false, s);  // builds the tables (warm-up)
+        execute(out, in, DFFT_BACKWARD, false, s);
+        check(hipEventRecord(e0, s), "hipEventRecord");
+        for (int rep = 0; rep < 3; ++rep) {
+          execute(in, out, DFFT_FORWARD, false, s);
+          execute(out, in, DFFT_BACKWARD, false, s);
+        }
+        check(hipEventRecord(e1, s), "hipEventRecord");
+        check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (best < 0.0f || ms < best) {
+          best = ms;
+          best_cap = cap;
+          best_mult = mult;
+          best_nt = nt;
+        }
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  set_option("lds_cap", static_cast<long>(best_cap));
+  set_option("grid_mult", best_mult);
+  set_option("nt", best_nt);
 }
 
 void Group::execute(void* const* in, void* const* out, int direction, bool use_graph, hipStream_t s) {

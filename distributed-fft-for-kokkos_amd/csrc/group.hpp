@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <string>
 #include <vector>
 
 #include "dfft.h"
@@ -28,6 +29,15 @@ class Group {
   int count() const { return static_cast<int>(members_.size()); }
   // launches one execute of `direction` issues
   int num_launches(int direction) const;
+  // "lds_cap": LDS bytes per block of the grouped kernel (lines per block
+  // shrink to fit; 1024..65536, default 65536); "grid_mult": resident-block
+  // multiples of the persistent grid (0: one block per virtual block);
+  // "nt": nontemporal global accesses (-1: on when a pass moves >= 256 MiB)
+  void set_option(const std::string& key, long value);
+  long get_option(const std::string& key) const;
+  // time LDS caps and grid sizes on the caller's arrays (contents are
+  // overwritten by forward + backward round trips) and keep the fastest
+  void autotune(void* const* in, void* const* out, hipStream_t s);
 
  private:
   struct Member {
@@ -42,11 +52,15 @@ class Group {
                                    int k) const;
   void upload(const std::vector<void*>& in, const std::vector<void*>& out, int dir, hipStream_t s);
   void launch_all(int dir, hipStream_t s);
+  void invalidate();
 
   std::vector<Member> members_;
   bool fp64_ = true;
   int device_ = 0;
   int max_passes_ = 0;
+  std::size_t lds_cap_ = kGroupLds;
+  int grid_mult_ = 1;
+  int nt_ = -1;
   std::map<int, void*> tw_;
   // per direction: the pointer set the device tables were built for, the
   // tables of every pass (LinePass items + virtual-block prefix), members

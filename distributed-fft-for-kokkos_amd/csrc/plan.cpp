@@ -340,6 +340,15 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   p.nlines = static_cast<int32_t>(nl);
   p.line_begin = 0;
   p.lines_fast = s.lines_fast >= 0 ? s.lines_fast : (s.axis != innermost ? 1 : 0);
+  if (s.nontemporal >= 0) {
+    p.nontemporal = s.nontemporal;
+  } else {
+    // streaming accesses pay off once the array no longer fits the caches
+    double elems = 1.0;
+    for (std::size_t a = 0; a < rank_; ++a) elems *= static_cast<double>(in.ext[a]);
+    const double es = (s.kind == kR2C ? 0.5 : 1.0) * (fp64_ ? 16.0 : 8.0);
+    p.nontemporal = elems * es >= 256.0 * 1024 * 1024 ? 1 : 0;
+  }
   return p;
 }
 
@@ -512,6 +521,7 @@ void Plan::set_option(const std::string& key, long value) {
   if (field == "lpb") passes[i].lines_per_block = static_cast<int>(value);
   else if (field == "vpt") passes[i].values_per_thread = static_cast<int>(value);
   else if (field == "lf") passes[i].lines_fast = static_cast<int>(value);
+  else if (field == "nt") passes[i].nontemporal = static_cast<int>(value);
   else throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -527,6 +537,7 @@ long Plan::get_option(const std::string& key) const {
   if (field == "lpb") return passes[i].lines_per_block;
   if (field == "vpt") return passes[i].values_per_thread;
   if (field == "lf") return passes[i].lines_fast;
+  if (field == "nt") return passes[i].nontemporal;
   throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -574,11 +585,12 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
       PassSpec best_spec = passes[i];
       for (int vpt : vpts) {
         for (int lpb : lpbs) {
-          for (int lf = 0; lf < 2; ++lf) {
+          for (int lfnt = 0; lfnt < 4; ++lfnt) {
             PassSpec s = passes[i];
             s.values_per_thread = vpt;
             s.lines_per_block = lpb;
-            s.lines_fast = lf;
+            s.lines_fast = lfnt & 1;
+            s.nontemporal = lfnt >> 1;
             LinePass p = make_pass(s, a, b, !forward, 1.0);
             if (launch_line_pass(p, fp64_, stream) != hipSuccess) {
               (void)hipGetLastError();

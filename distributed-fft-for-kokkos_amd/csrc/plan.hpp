@@ -23,6 +23,7 @@ struct PassSpec {
   int lines_per_block = 0;
   int values_per_thread = 0;
   int lines_fast = -1;
+  int nontemporal = -1;  // -1: on for arrays of at least 256 MiB
 };
 
 // device table exp(-2 pi i e / n), e in [0, n): evaluated in long double,
@@ -44,8 +45,8 @@ class Plan {
   std::vector<float> execute_timed(void* const* in, void* const* out, int direction, hipStream_t stream);
   int num_passes(int direction) const;
   // tuning knobs: "chunk" (planes per chunk of the first two passes, 0 = off),
-  // "fwd.<i>.lpb|vpt|lf", "bwd.<i>.lpb|vpt|lf" (lines per block, values per
-  // thread, lanes-across-lines) of pass i
+  // "fwd.<i>.lpb|vpt|lf|nt", "bwd.<i>.lpb|vpt|lf|nt" (lines per block, values
+  // per thread, lanes-across-lines, nontemporal accesses) of pass i
   void set_option(const std::string& key, long value);
   long get_option(const std::string& key) const;
   // measure candidate launch shapes and chunk sizes on the caller's arrays

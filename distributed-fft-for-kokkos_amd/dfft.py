@@ -162,7 +162,8 @@ EXPORTED = [
     "dfft_execute_timed", "dfft_plan_set_option", "dfft_plan_get_option", "dfft_plan_autotune", "dfft_pack",
     "dfft_unpack", "dfft_all2all", "dfft_exchange", "dfft_get_all_slab_topologies",
     "dfft_get_all_pencil_topologies", "dfft_block_analyses", "dfft_group_create", "dfft_group_destroy",
-    "dfft_group_execute", "dfft_group_num_launches", "dfft_ns_create", "dfft_ns_destroy",
+    "dfft_group_execute", "dfft_group_num_launches", "dfft_group_set_option", "dfft_group_get_option",
+    "dfft_group_autotune", "dfft_ns_create", "dfft_ns_destroy",
     "dfft_ns_local_extents", "dfft_ns_init_taylor_green", "dfft_ns_set_physical", "dfft_ns_set_state",
     "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_energy", "dfft_ns_velocity", "dfft_ns_vorticity",
     "dfft_ns_energy_spectrum", "dfft_ns_diag", "dfft_ns_rhs", "dfft_ns_rhs_tangent",
@@ -222,6 +223,9 @@ def load_library(path: os.PathLike | None = None):
         "dfft_group_execute": ([ctypes.c_void_p, vpp, vpp, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
                                ctypes.c_int),
         "dfft_group_num_launches": ([ctypes.c_void_p, ctypes.c_int, ip], ctypes.c_int),
+        "dfft_group_set_option": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long], ctypes.c_int),
+        "dfft_group_get_option": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "dfft_group_autotune": ([ctypes.c_void_p, vpp, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_create": ([ctypes.POINTER(NsDesc), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
         "dfft_ns_destroy": ([ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_local_extents": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, szp, szp], ctypes.c_int),
@@ -493,7 +497,7 @@ class Plan:
         out = {"fwd.chunk": self.get_option("fwd.chunk"), "bwd.chunk": self.get_option("bwd.chunk")}
         for d, dirn in (("fwd", Direction.forward), ("bwd", Direction.backward)):
             for i in range(max(0, self.num_passes(dirn))):
-                for f in ("lpb", "vpt", "lf"):
+                for f in ("lpb", "vpt", "lf", "nt"):
                     out[f"{d}.{i}.{f}"] = self.get_option(f"{d}.{i}.{f}")
         return out
 
@@ -551,6 +555,18 @@ class Group:
         v = ctypes.c_int()
         _check(self._lib.dfft_group_num_launches(self._h, int(direction), ctypes.byref(v)))
         return int(v.value)
+
+    def set_option(self, key, value):
+        _check(self._lib.dfft_group_set_option(self._h, key.encode(), int(value)))
+
+    def get_option(self, key):
+        v = ctypes.c_long()
+        _check(self._lib.dfft_group_get_option(self._h, key.encode(), ctypes.byref(v)))
+        return int(v.value)
+
+    def autotune(self, ins, outs, stream=None):
+        """Time LDS caps and grid sizes on these arrays (their contents are overwritten)."""
+        _check(self._lib.dfft_group_autotune(self._h, _ptrs(ins), _ptrs(outs), _stream_handle(stream)))
 
     def destroy(self):
         if getattr(self, "_h", None) and self._h.value:

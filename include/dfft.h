@@ -144,8 +144,9 @@ int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int d
 /* Launch-shape knobs of a fused plan (no counterpart in the reference, whose
  * vendor FFT plans choose their own):  "chunk" / "fwd.chunk" / "bwd.chunk" =
  * planes per chunk of the first two passes (0 = off); "fwd.<i>.lpb",
- * "fwd.<i>.vpt", "fwd.<i>.lf" (and "bwd.…") = lines per block, values per
- * thread (4, 8, 16) and lane order of pass i. */
+ * "fwd.<i>.vpt", "fwd.<i>.lf", "fwd.<i>.nt" (and "bwd.…") = lines per block,
+ * values per thread (4, 8, 16), lane order and nontemporal global accesses
+ * (-1 = on for arrays of at least 256 MiB) of pass i. */
 int dfft_plan_set_option(dfft_plan* plan, const char* key, long value);
 int dfft_plan_get_option(const dfft_plan* plan, const char* key, long* value);
 /* Autotuner: times every launch shape of every pass and the chunk sizes on the
@@ -190,6 +191,13 @@ int dfft_group_destroy(dfft_group* group);
 int dfft_group_execute(dfft_group* group, void* const* in, void* const* out, int direction, int use_graph,
                        void* stream);
 int dfft_group_num_launches(const dfft_group* group, int direction, int* nlaunches);
+/* "lds_cap" (LDS bytes per block of the grouped kernel, 1024..65536),
+ * "grid_mult" (multiples of the resident-block grid; 0 = one block per
+ * virtual block) and "nt" (nontemporal accesses: -1 auto, 0, 1); the autotuner times combinations on the caller's arrays
+ * (contents are overwritten) and keeps the fastest. */
+int dfft_group_set_option(dfft_group* group, const char* key, long value);
+int dfft_group_get_option(const dfft_group* group, const char* key, long* value);
+int dfft_group_autotune(dfft_group* group, void* const* in, void* const* out, void* stream);
 
 /* ---- pseudo-spectral Navier-Stokes (examples/navier-stokes-MPI) -------- */
 /* The example's solver (main.cpp:31-986, utils/math_utils.hpp:47-141) on P =

@@ -239,7 +239,8 @@ def test_diagnostics_of_taylor_green(dev, tmp_path):
     hit = int(np.sqrt(3.0) // (k_max / 50))
     coef = (2 * np.pi) ** 3 / n ** 6
     assert abs(e_spec[hit] * coef - np.pi ** 3) < 1e-12 * np.pi ** 3
-    assert np.abs(np.delete(e_spec, hit)).max() == 0.0
+    # other bins hold only the squares of transform round-off
+    assert np.abs(np.delete(e_spec, hit)).max() < 1e-20 * e_spec[hit]
 
 
 def test_energy_spectrum_sums_to_energy(dev):
