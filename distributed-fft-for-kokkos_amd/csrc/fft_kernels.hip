@@ -140,28 +140,31 @@ template <int r, typename C> __device__ __forceinline__ void bfly(C* a) {
 // ---------------------------------------------------------------------------
 // Element access of a line.
 // ---------------------------------------------------------------------------
-// Global element access: nontemporal (streaming) or cached.
-template <typename T> __device__ __forceinline__ T gload(const T* p, bool nt) {
-  return nt ? __builtin_nontemporal_load(p) : *p;
+// Global element access: nontemporal (streaming) or cached, chosen at
+// compile time (a runtime choice between the two forms of one access is
+// folded into the cached form by the compiler).
+template <bool NT, typename T> __device__ __forceinline__ T gload(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
 }
-template <typename T> __device__ __forceinline__ void gstore(T* p, T v, bool nt) {
-  if (nt) __builtin_nontemporal_store(v, p);
+template <bool NT, typename T> __device__ __forceinline__ void gstore(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
-template <typename Real> __device__ __forceinline__ cplx_t<Real> gload_c(const cplx_t<Real>* p, bool nt) {
+template <bool NT, typename Real> __device__ __forceinline__ cplx_t<Real> gload_c(const cplx_t<Real>* p) {
   typedef Real V2 __attribute__((ext_vector_type(2)));
-  const V2 t = gload(reinterpret_cast<const V2*>(p), nt);
+  const V2 t = gload<NT>(reinterpret_cast<const V2*>(p));
   return cplx_t<Real>{t.x, t.y};
 }
-template <typename Real> __device__ __forceinline__ void gstore_c(cplx_t<Real>* p, cplx_t<Real> v, bool nt) {
+template <bool NT, typename Real> __device__ __forceinline__ void gstore_c(cplx_t<Real>* p, cplx_t<Real> v) {
   typedef Real V2 __attribute__((ext_vector_type(2)));
   V2 t;
   t.x = v.x;
   t.y = v.y;
-  gstore(reinterpret_cast<V2*>(p), t, nt);
+  gstore<NT>(reinterpret_cast<V2*>(p), t);
 }
 
-template <typename Real> struct LineIO {
+template <typename Real, bool NT = false> struct LineIO {
   using C = cplx_t<Real>;
   const char* ib;
   char* ob;
@@ -169,20 +172,19 @@ template <typename Real> struct LineIO {
   int32_t kind, n, inverse;
   Real scale;
   Real hw;  // interior half-spectrum weight (1: none)
-  bool nt;  // nontemporal global accesses
 
   __device__ __forceinline__ C load(int32_t pos) const {
     C v;
     if (kind == kC2C) {
-      v = gload_c<Real>(reinterpret_cast<const C*>(ib) + pos * is, nt);
+      v = gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + pos * is);
     } else if (kind == kR2C) {
-      v = C{gload(reinterpret_cast<const Real*>(ib) + pos * is, nt), Real(0)};
+      v = C{gload<NT>(reinterpret_cast<const Real*>(ib) + pos * is), Real(0)};
     } else {
       const int32_t nh = n / 2 + 1;
       if (pos < nh) {
-        v = gload_c<Real>(reinterpret_cast<const C*>(ib) + pos * is, nt);
+        v = gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + pos * is);
       } else {
-        v = cconj(gload_c<Real>(reinterpret_cast<const C*>(ib) + (n - pos) * is, nt));
+        v = cconj(gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + (n - pos) * is));
       }
       if (pos == 0 || 2 * pos == n) {
         v.y = Real(0);
@@ -198,15 +200,75 @@ template <typename Real> struct LineIO {
     v.x *= scale;
     v.y *= scale;
     if (kind == kC2C) {
-      gstore_c<Real>(reinterpret_cast<C*>(ob) + pos * os, v, nt);
+      gstore_c<NT, Real>(reinterpret_cast<C*>(ob) + pos * os, v);
     } else if (kind == kR2C) {
       if (pos != 0 && 2 * pos != n) {
         v.x *= hw;
         v.y *= hw;
       }
-      if (2 * pos <= n) gstore_c<Real>(reinterpret_cast<C*>(ob) + pos * os, v, nt);
+      if (2 * pos <= n) gstore_c<NT, Real>(reinterpret_cast<C*>(ob) + pos * os, v);
     } else {
-      gstore(reinterpret_cast<Real*>(ob) + pos * os, v.x, nt);
+      gstore<NT>(reinterpret_cast<Real*>(ob) + pos * os, v.x);
+    }
+  }
+};
+
+// Two real lines a, b per complex line.  R2C: z = a + i b, and after the
+// forward FFT Z the spectra are A_k = (Z_k + conj Z_{n-k}) / 2 and
+// B_k = (Z_k - conj Z_{n-k}) / (2i).  C2R: Z = A + i B over the Hermitian
+// extensions of both half spectra; the inverse transform of Z is a + i b.
+template <typename Real, bool NT = false> struct PairIO {
+  using C = cplx_t<Real>;
+  const char* ib[2];
+  char* ob[2];
+  int64_t is[2], os[2];  // per line: the two lines may lie in different rank blocks
+  int32_t kind, n;
+  Real scale;
+  Real hw;     // interior half-spectrum weight (1: none)
+  bool has_b;  // the pair's second line exists
+
+  // value fed to the forward FFT at position pos (C2R lines are conjugated:
+  // inverse = conj o FFT o conj)
+  __device__ __forceinline__ C load(int32_t pos) const {
+    if (kind == kR2C) {
+      const Real a = gload<NT>(reinterpret_cast<const Real*>(ib[0]) + pos * is[0]);
+      const Real b = has_b ? gload<NT>(reinterpret_cast<const Real*>(ib[1]) + pos * is[1]) : Real(0);
+      return C{a, b};
+    }
+    const int32_t nh = n / 2 + 1;
+    const bool lo = pos < nh;
+    const int32_t src = lo ? pos : n - pos;
+    C A = gload_c<NT, Real>(reinterpret_cast<const C*>(ib[0]) + src * is[0]);
+    C Bv = has_b ? gload_c<NT, Real>(reinterpret_cast<const C*>(ib[1]) + src * is[1]) : C{Real(0), Real(0)};
+    if (!lo) {
+      A = cconj(A);
+      Bv = cconj(Bv);
+    }
+    if (pos == 0 || 2 * pos == n) {
+      A.y = Real(0);
+      Bv.y = Real(0);
+    } else {
+      A.x *= hw;
+      A.y *= hw;
+      Bv.x *= hw;
+      Bv.y *= hw;
+    }
+    // conj(A + i B)
+    return C{A.x - Bv.y, -(A.y + Bv.x)};
+  }
+  // C2R: w = FFT(conj Z) at position pos, so a + i b = conj(w)
+  __device__ __forceinline__ void store_c2r(int32_t pos, C w) const {
+    gstore<NT>(reinterpret_cast<Real*>(ob[0]) + pos * os[0], w.x * scale);
+    if (has_b) gstore<NT>(reinterpret_cast<Real*>(ob[1]) + pos * os[1], -w.y * scale);
+  }
+  // R2C: spectra of both lines at pos <= n/2 from Z_pos and Z_{n-pos}
+  __device__ __forceinline__ void store_r2c(int32_t pos, C zk, C zm) const {
+    const Real h = Real(0.5) * scale * ((pos != 0 && 2 * pos != n) ? hw : Real(1));
+    const C A{(zk.x + zm.x) * h, (zk.y - zm.y) * h};
+    gstore_c<NT, Real>(reinterpret_cast<C*>(ob[0]) + pos * os[0], A);
+    if (has_b) {
+      const C Bv{(zk.y + zm.y) * h, (zm.x - zk.x) * h};
+      gstore_c<NT, Real>(reinterpret_cast<C*>(ob[1]) + pos * os[1], Bv);
     }
   }
 };
@@ -226,12 +288,12 @@ __device__ __forceinline__ void decode_line(const LinePass& p, int32_t line, int
   }
 }
 
-template <typename Real>
-__device__ __forceinline__ LineIO<Real> make_line_io(const LinePass& p, int32_t line) {
+template <typename Real, bool NT = false>
+__device__ __forceinline__ LineIO<Real, NT> make_line_io(const LinePass& p, int32_t line) {
   using C = cplx_t<Real>;
   int32_t idx[kMaxDim];
   decode_line<Real>(p, line, idx);
-  LineIO<Real> io;
+  LineIO<Real, NT> io;
   const int64_t ies = p.kind == kR2C ? sizeof(Real) : sizeof(C);
   const int64_t oes = p.kind == kC2R ? sizeof(Real) : sizeof(C);
   io.ib = element_ptr(p.in, idx, ies, p.axis, io.is);
@@ -241,7 +303,35 @@ __device__ __forceinline__ LineIO<Real> make_line_io(const LinePass& p, int32_t 
   io.inverse = p.inverse;
   io.scale = static_cast<Real>(p.scale);
   io.hw = p.herm_weight != 0.0 ? static_cast<Real>(p.herm_weight) : Real(1);
-  io.nt = p.nontemporal != 0;
+  return io;
+}
+
+template <typename Real, bool NT>
+__device__ __forceinline__ PairIO<Real, NT> make_pair_io(const LinePass& p, int32_t pair) {
+  using C = cplx_t<Real>;
+  PairIO<Real, NT> io;
+  const int64_t ies = p.kind == kR2C ? sizeof(Real) : sizeof(C);
+  const int64_t oes = p.kind == kC2R ? sizeof(Real) : sizeof(C);
+  int32_t idx[kMaxDim];
+  const int32_t la = 2 * pair;
+  decode_line<Real>(p, la, idx);
+  io.ib[0] = element_ptr(p.in, idx, ies, p.axis, io.is[0]);
+  io.ob[0] = element_ptr(p.out, idx, oes, p.axis, io.os[0]);
+  io.has_b = la + 1 < p.nlines;
+  if (io.has_b) {
+    decode_line<Real>(p, la + 1, idx);
+    io.ib[1] = element_ptr(p.in, idx, ies, p.axis, io.is[1]);
+    io.ob[1] = element_ptr(p.out, idx, oes, p.axis, io.os[1]);
+  } else {
+    io.ib[1] = io.ib[0];
+    io.ob[1] = io.ob[0];
+    io.is[1] = io.is[0];
+    io.os[1] = io.os[0];
+  }
+  io.kind = p.kind;
+  io.n = p.n;
+  io.scale = static_cast<Real>(p.scale);
+  io.hw = p.herm_weight != 0.0 ? static_cast<Real>(p.herm_weight) : Real(1);
   return io;
 }
 
@@ -349,7 +439,7 @@ template <typename Real, int N, int RM, int S> struct StageLoop {
 // lanes per line.  Lanes beyond B*T (a block shape shared by several line
 // lengths) run the stages on zeros in a spare LDS slot (index B) so every
 // lane reaches every barrier; they load and store nothing.
-template <typename Real, int N, int RM>
+template <typename Real, int N, int RM, bool NT>
 __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, int B, cplx_t<Real>* lds) {
   using C = cplx_t<Real>;
   using Cfg = FastCfg<Real, N, RM>;
@@ -373,8 +463,8 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
   const double2* tw = reinterpret_cast<const double2*>(p.tw);
   C* line_lds = lds + b * Cfg::LDSN;
 
-  LineIO<Real> io;
-  if (valid) io = make_line_io<Real>(p, line);
+  LineIO<Real, NT> io;
+  if (valid) io = make_line_io<Real, NT>(p, line);
 
   C v[R];
   {
@@ -410,10 +500,93 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
   }
 }
 
-template <typename Real, int N, int RM>
+// Paired R2C / C2R: slot b of the block transforms lines 2q and 2q+1
+// (q = blk*B + b) as one complex line; same stage code as fast_line_body.
+template <typename Real, int N, int RM, bool NT>
+__device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, int B, cplx_t<Real>* lds) {
+  using C = cplx_t<Real>;
+  using Cfg = FastCfg<Real, N, RM>;
+  constexpr int R = Cfg::R, T = Cfg::T, NST = Cfg::NST;
+  const int tid = threadIdx.x;
+  int b, t;
+  if (p.lines_fast) {
+    b = tid % B;
+    t = tid / B;
+  } else {
+    t = tid % T;
+    b = tid / T;
+  }
+  const bool active = tid < B * T;
+  if (!active) {
+    b = B;
+    t = 0;
+  }
+  const int32_t pair = blk * B + b;
+  const bool valid = active && 2 * pair < p.nlines;
+  const double2* tw = reinterpret_cast<const double2*>(p.tw);
+  C* line_lds = lds + b * Cfg::LDSN;
+
+  PairIO<Real, NT> io;
+  if (valid) io = make_pair_io<Real, NT>(p, pair);
+
+  C v[R];
+  {
+    constexpr int r0 = Cfg::P.rad[0];
+    constexpr int NB0 = R / r0;
+#pragma unroll
+    for (int q = 0; q < NB0; ++q) {
+      const int beta = t * NB0 + q;
+#pragma unroll
+      for (int j = 0; j < r0; ++j) v[q * r0 + j] = valid ? io.load(beta + j * (N / r0)) : C{Real(0), Real(0)};
+    }
+  }
+  StageLoop<Real, N, RM, 0>::run(v, t, tw, line_lds);
+
+  constexpr int rl = Cfg::P.rad[NST - 1];
+  if constexpr (rl != R) {
+    constexpr int NBl = R / rl;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NBl; ++q) {
+      const int beta = t * NBl + q;
+#pragma unroll
+      for (int j = 0; j < rl; ++j) line_lds[Cfg::lidx(beta + j * (N / rl))] = v[q * rl + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = line_lds[Cfg::lidx(t + j * T)];
+  }
+  if (p.kind == kC2R) {
+    if (valid) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) io.store_c2r(t + j * T, v[j]);
+    }
+    return;
+  }
+  // R2C: Z_{n-pos} lives in another lane; exchange the whole line through LDS
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R; ++j) line_lds[Cfg::lidx(t + j * T)] = v[j];
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int pos = t + j * T;
+      if (2 * pos <= N) io.store_r2c(pos, v[j], line_lds[Cfg::lidx(pos == 0 ? 0 : N - pos)]);
+    }
+  }
+}
+
+template <typename Real, int N, int RM, bool NT>
+__global__ void __launch_bounds__(1024) fft_pair_fast(const LinePass p) {
+  extern __shared__ unsigned char smem_raw[];
+  fast_pair_body<Real, N, RM, NT>(p, blockIdx.x, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
+}
+
+template <typename Real, int N, int RM, bool NT>
 __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
-  fast_line_body<Real, N, RM>(p, blockIdx.x, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
+  fast_line_body<Real, N, RM, NT>(p, blockIdx.x, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
 }
 
 // ---------------------------------------------------------------------------
@@ -423,7 +596,7 @@ __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
 // most a few blocks per CU walks them in a block-stride loop.  Every block
 // has kGroupThreads lanes; transform i uses items[i].lines_per_block lines.
 // ---------------------------------------------------------------------------
-template <typename Real>
+template <typename Real, bool NT>
 __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* __restrict__ items,
                                                                 const int32_t* __restrict__ first_block,
                                                                 int32_t count, int32_t total_blocks) {
@@ -441,13 +614,13 @@ __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* 
     const LinePass& p = items[i];
     const int32_t blk = vb - first_block[i];
     switch (p.n) {
-      case 16: fast_line_body<Real, 16, 8>(p, blk, p.lines_per_block, lds); break;
-      case 32: fast_line_body<Real, 32, 8>(p, blk, p.lines_per_block, lds); break;
-      case 64: fast_line_body<Real, 64, 8>(p, blk, p.lines_per_block, lds); break;
-      case 96: fast_line_body<Real, 96, 8>(p, blk, p.lines_per_block, lds); break;
-      case 128: fast_line_body<Real, 128, 8>(p, blk, p.lines_per_block, lds); break;
-      case 160: fast_line_body<Real, 160, 8>(p, blk, p.lines_per_block, lds); break;
-      case 256: fast_line_body<Real, 256, 8>(p, blk, p.lines_per_block, lds); break;
+      case 16: fast_line_body<Real, 16, 8, NT>(p, blk, p.lines_per_block, lds); break;
+      case 32: fast_line_body<Real, 32, 8, NT>(p, blk, p.lines_per_block, lds); break;
+      case 64: fast_line_body<Real, 64, 8, NT>(p, blk, p.lines_per_block, lds); break;
+      case 96: fast_line_body<Real, 96, 8, NT>(p, blk, p.lines_per_block, lds); break;
+      case 128: fast_line_body<Real, 128, 8, NT>(p, blk, p.lines_per_block, lds); break;
+      case 160: fast_line_body<Real, 160, 8, NT>(p, blk, p.lines_per_block, lds); break;
+      case 256: fast_line_body<Real, 256, 8, NT>(p, blk, p.lines_per_block, lds); break;
       default: break;
     }
     __syncthreads();  // the next virtual block reuses the LDS lines
@@ -490,7 +663,6 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
       io.inverse = p.inverse;
       io.scale = static_cast<Real>(p.scale);
       io.hw = p.herm_weight != 0.0 ? static_cast<Real>(p.herm_weight) : Real(1);
-      io.nt = false;
       if (p.kind == kC2R && pos >= n / 2 + 1) {
         idx[p.axis] = n - pos;
       } else {
@@ -562,7 +734,6 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
     io.inverse = p.inverse;
     io.scale = static_cast<Real>(p.scale);
     io.hw = Real(1);
-    io.nt = false;
     io.ob = element_ptr(p.out, idx, oes, p.axis, st);
     io.os = 0;
     // store(0, .) sees position 0, so the interior weight is applied here
@@ -598,18 +769,32 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
   const size_t smem = static_cast<size_t>(B) * Cfg::LDSN * sizeof(C);
   if (B * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
   p.lines_per_block = B;
+  // two real lines per complex FFT for forward R2C and inverse C2R passes
+  // (kernels for 8 values per thread)
+  const int pair = RM == 8 && p.paired && ((p.kind == kR2C && !p.inverse) || (p.kind == kC2R && p.inverse)) ? 1 : 0;
+  const int nt = p.nontemporal ? 1 : 0;
+  const void* fns[4] = {reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM, false>),
+                        reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM, true>),
+                        reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, false>),
+                        reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, true>)};
+  const int v = 2 * pair + nt;
   // raise the dynamic-LDS limit once per kernel instance (never inside a
   // stream capture that follows an eager launch of the same shape)
-  static size_t lds_limit = 64 * 1024;
-  if (smem > lds_limit) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
+  static size_t lds_limit[4] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
+  if (smem > lds_limit[v]) {
+    hipError_t e = hipFuncSetAttribute(fns[v], hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
     if (e != hipSuccess) return e;
-    lds_limit = smem;
+    lds_limit[v] = smem;
   }
-  const unsigned grid = static_cast<unsigned>((p.nlines + B - 1) / B);
+  const int64_t units = pair ? (static_cast<int64_t>(p.nlines) + 1) / 2 : p.nlines;
+  const unsigned grid = static_cast<unsigned>((units + B - 1) / B);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((fft_line_fast<Real, N, RM>), dim3(grid), dim3(T * B), smem, s, p);
+  switch (v) {
+    case 0: hipLaunchKernelGGL((fft_line_fast<Real, N, RM, false>), dim3(grid), dim3(T * B), smem, s, p); break;
+    case 1: hipLaunchKernelGGL((fft_line_fast<Real, N, RM, true>), dim3(grid), dim3(T * B), smem, s, p); break;
+    case 2: hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, false>), dim3(grid), dim3(T * B), smem, s, p); break;
+    default: hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, true>), dim3(grid), dim3(T * B), smem, s, p); break;
+  }
   return hipGetLastError();
 }
 
@@ -716,23 +901,29 @@ int group_grid(size_t lds_bytes, bool fp64) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  const void* f = fp64 ? reinterpret_cast<const void*>(&fft_group_pass<double>)
-                       : reinterpret_cast<const void*>(&fft_group_pass<float>);
+  const void* f = fp64 ? reinterpret_cast<const void*>(&fft_group_pass<double, false>)
+                       : reinterpret_cast<const void*>(&fft_group_pass<float, false>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kGroupThreads, lds_bytes) != hipSuccess) return 0;
   return cus * (per_cu > 0 ? per_cu : 1);
 }
 
 hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
-                             int grid, size_t lds_bytes, bool fp64, hipStream_t s) {
+                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, hipStream_t s) {
   if (total_blocks <= 0 || count <= 0) return hipSuccess;
   if (grid <= 0 || lds_bytes > kGroupLds) return hipErrorInvalidConfiguration;
-  const unsigned g = static_cast<unsigned>(grid < total_blocks ? grid : total_blocks);
-  if (fp64)
-    hipLaunchKernelGGL(fft_group_pass<double>, dim3(g), dim3(kGroupThreads), lds_bytes, s, items, first_block, count,
-                       total_blocks);
-  else
-    hipLaunchKernelGGL(fft_group_pass<float>, dim3(g), dim3(kGroupThreads), lds_bytes, s, items, first_block, count,
-                       total_blocks);
+  const dim3 g(static_cast<unsigned>(grid < total_blocks ? grid : total_blocks));
+  const dim3 b(kGroupThreads);
+  if (fp64) {
+    if (nontemporal)
+      hipLaunchKernelGGL((fft_group_pass<double, true>), g, b, lds_bytes, s, items, first_block, count, total_blocks);
+    else
+      hipLaunchKernelGGL((fft_group_pass<double, false>), g, b, lds_bytes, s, items, first_block, count, total_blocks);
+  } else {
+    if (nontemporal)
+      hipLaunchKernelGGL((fft_group_pass<float, true>), g, b, lds_bytes, s, items, first_block, count, total_blocks);
+    else
+      hipLaunchKernelGGL((fft_group_pass<float, false>), g, b, lds_bytes, s, items, first_block, count, total_blocks);
+  }
   return hipGetLastError();
 }
 

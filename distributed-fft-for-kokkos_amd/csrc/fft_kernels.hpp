@@ -31,6 +31,9 @@ struct LinePass {
   int32_t values_per_thread;            // 0: default (8); compile-time kernels exist for 4, 8, 16
   const void* tw;                       // exp(-2 pi i e / n), e in [0, n), Real precision
   double scale;
+  // 1: an R2C (forward) or C2R (inverse) pass on the fast kernel transforms
+  // two real lines per complex FFT (lines 2q and 2q+1 of the pass)
+  int32_t paired;
   // 1: global loads and stores of the line data are nontemporal (streaming
   // arrays larger than the caches; measured 5-13 % faster per pass at 512^3)
   int32_t nontemporal;
@@ -61,7 +64,7 @@ bool group_line_shape(int n, bool fp64, size_t lds_cap, int* lines_per_block, si
 // grid of the persistent grouped launch: resident blocks of the whole device
 int group_grid(size_t lds_bytes, bool fp64);
 hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
-                             int grid, size_t lds_bytes, bool fp64, hipStream_t s);
+                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, hipStream_t s);
 bool has_fast_line_kernel(int n);
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
 hipError_t launch_remap(const RemapArgs& a, int esize, hipStream_t s);

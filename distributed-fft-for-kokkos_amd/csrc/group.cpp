@@ -201,6 +201,7 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
   ds.count.assign(np, 0);
   ds.total.assign(np, 0);
   ds.grid.assign(np, 0);
+  ds.nt.assign(np, 0);
   ds.lds.assign(np, 0);
   ds.single.assign(np, {});
   for (std::size_t k = 0; k < np; ++k) {
@@ -213,6 +214,7 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
     }
     const int nt = nt_ >= 0 ? nt_ : (bytes >= 256.0 * 1024 * 1024 ? 1 : 0);
     for (LinePass& p : all) p.nontemporal = nt;
+    ds.nt[k] = nt;
     std::vector<LinePass> grouped;
     std::vector<int32_t> first{0};
     std::size_t lds = 0;
@@ -257,7 +259,8 @@ void Group::launch_all(int dir, hipStream_t s) {
   const DirState& ds = dir_[dir];
   for (int k = 0; k < max_passes_; ++k) {
     if (ds.count[k] > 0)
-      check(launch_group_pass(ds.items[k], ds.first[k], ds.count[k], ds.total[k], ds.grid[k], ds.lds[k], fp64_, s),
+      check(launch_group_pass(ds.items[k], ds.first[k], ds.count[k], ds.total[k], ds.grid[k], ds.lds[k], fp64_,
+                              ds.nt[k] != 0, s),
             "grouped line pass");
     for (const LinePass& p : ds.single[k]) check(launch_line_pass(p, fp64_, s), "line pass");
   }

@@ -70,7 +70,7 @@ class Group {
     bool built = false;
     std::vector<LinePass*> items;     // device, one array per pass
     std::vector<int32_t*> first;      // device
-    std::vector<int> count, total, grid;
+    std::vector<int> count, total, grid, nt;
     std::vector<size_t> lds;
     std::vector<std::vector<LinePass>> single;  // host: individually launched passes
     hipGraph_t graph = nullptr;

@@ -310,6 +310,7 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   p.inverse = inverse ? 1 : 0;
   p.scale = scale;
   p.herm_weight = s.kind == kC2C ? 0.0 : adj_herm_;
+  p.paired = s.kind == kC2C ? 0 : s.paired;
   p.tw = twiddles(s.n);
   p.lines_per_block = s.lines_per_block;
   p.values_per_thread = s.values_per_thread;
@@ -522,6 +523,7 @@ void Plan::set_option(const std::string& key, long value) {
   else if (field == "vpt") passes[i].values_per_thread = static_cast<int>(value);
   else if (field == "lf") passes[i].lines_fast = static_cast<int>(value);
   else if (field == "nt") passes[i].nontemporal = static_cast<int>(value);
+  else if (field == "pair") passes[i].paired = static_cast<int>(value);
   else throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -538,6 +540,7 @@ long Plan::get_option(const std::string& key) const {
   if (field == "vpt") return passes[i].values_per_thread;
   if (field == "lf") return passes[i].lines_fast;
   if (field == "nt") return passes[i].nontemporal;
+  if (field == "pair") return passes[i].paired;
   throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -585,12 +588,13 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
       PassSpec best_spec = passes[i];
       for (int vpt : vpts) {
         for (int lpb : lpbs) {
-          for (int lfnt = 0; lfnt < 4; ++lfnt) {
+          for (int lfnt = 0; lfnt < (passes[i].kind == kC2C ? 4 : 8); ++lfnt) {
             PassSpec s = passes[i];
             s.values_per_thread = vpt;
             s.lines_per_block = lpb;
             s.lines_fast = lfnt & 1;
-            s.nontemporal = lfnt >> 1;
+            s.nontemporal = (lfnt >> 1) & 1;
+            s.paired = lfnt >> 2;
             LinePass p = make_pass(s, a, b, !forward, 1.0);
             if (launch_line_pass(p, fp64_, stream) != hipSuccess) {
               (void)hipGetLastError();
