@@ -120,6 +120,7 @@ def bench_ns(torch, steps):
     import dfft_ns
     n = 256
     s = dfft_ns.NavierStokes(n, 2, 2, lx=2.0, nu=1.0 / 1600.0, dt=1e-3, form=dfft_ns.DIVERGENCE)
+    s.autotune()
     s.init_taylor_green(1.0)
     e0 = s.energy()
     s.step(2, use_graph=True)  # eager first step + capture
@@ -144,6 +145,7 @@ def bench_ns_adjoint(torch, dev):
     u0 = (u0 + 1e-2 * torch.randn(u0.shape, generator=gen, device=dev, dtype=torch.float64)).contiguous()
     s = dfft_ns.NavierStokes(n, 1, 1, lx=2.0, nu=1.0 / 1600.0, dt=1e-3, form=dfft_ns.DIVERGENCE)
     u0d = [u0]
+    s.autotune()
     s.gradient(2, u0d, checkpoint_every=1)  # warm-up (allocations, kernel setup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -492,6 +492,13 @@ int dfft_ns_step(dfft_ns* ns, int nsteps, int use_graph, void* stream) {
   });
 }
 
+int dfft_ns_autotune(dfft_ns* ns, void* stream) {
+  return guarded([&] {
+    need(ns, "null pointer");
+    ns->impl->autotune(static_cast<hipStream_t>(stream));
+  });
+}
+
 int dfft_ns_energy(dfft_ns* ns, double* energy, void* stream) {
   return guarded([&] {
     need(ns && energy, "null pointer");

@@ -577,16 +577,27 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
   }
 }
 
+// Workgroups go to the 8 XCDs round-robin; renumber them so XCD x owns the
+// x-th contiguous range of blocks (a bijection for any grid size).
+__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nb) {
+  constexpr int32_t kXcds = 8;
+  const int32_t q = nb / kXcds, r = nb % kXcds;
+  const int32_t x = b % kXcds, i = b / kXcds;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 template <typename Real, int N, int RM, bool NT>
 __global__ void __launch_bounds__(1024) fft_pair_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
-  fast_pair_body<Real, N, RM, NT>(p, blockIdx.x, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
+  const int32_t blk = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : static_cast<int32_t>(blockIdx.x);
+  fast_pair_body<Real, N, RM, NT>(p, blk, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
 }
 
 template <typename Real, int N, int RM, bool NT>
 __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
-  fast_line_body<Real, N, RM, NT>(p, blockIdx.x, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
+  const int32_t blk = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : static_cast<int32_t>(blockIdx.x);
+  fast_line_body<Real, N, RM, NT>(p, blk, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
 }
 
 // ---------------------------------------------------------------------------

@@ -34,6 +34,9 @@ struct LinePass {
   // 1: an R2C (forward) or C2R (inverse) pass on the fast kernel transforms
   // two real lines per complex FFT (lines 2q and 2q+1 of the pass)
   int32_t paired;
+  // 1: blocks are renumbered so each of the 8 XCDs (which take workgroups
+  // round-robin) walks one contiguous range of lines
+  int32_t xcd_remap;
   // 1: global loads and stores of the line data are nontemporal (streaming
   // arrays larger than the caches; measured 5-13 % faster per pass at 512^3)
   int32_t nontemporal;

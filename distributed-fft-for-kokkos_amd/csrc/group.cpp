@@ -212,7 +212,7 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
       for (int a = 0; a < p.in.ndim; ++a) e *= static_cast<double>(p.in.ext[a]);
       bytes += e * (p.kind == kR2C ? 0.5 : 1.0) * (fp64_ ? 16.0 : 8.0);
     }
-    const int nt = nt_ >= 0 ? nt_ : (bytes >= 256.0 * 1024 * 1024 ? 1 : 0);
+    const int nt = nt_ >= 0 ? nt_ : (bytes >= 1024.0 * 1024 * 1024 ? 1 : 0);
     for (LinePass& p : all) p.nontemporal = nt;
     ds.nt[k] = nt;
     std::vector<LinePass> grouped;

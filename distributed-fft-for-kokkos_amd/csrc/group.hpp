@@ -32,7 +32,7 @@ class Group {
   // "lds_cap": LDS bytes per block of the grouped kernel (lines per block
   // shrink to fit; 1024..65536, default 65536); "grid_mult": resident-block
   // multiples of the persistent grid (0: one block per virtual block);
-  // "nt": nontemporal global accesses (-1: on when a pass moves >= 256 MiB)
+  // "nt": nontemporal global accesses (-1: on when a pass moves >= 1 GiB)
   void set_option(const std::string& key, long value);
   long get_option(const std::string& key) const;
   // time LDS caps and grid sizes on the caller's arrays (contents are

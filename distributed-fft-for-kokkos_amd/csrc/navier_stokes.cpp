@@ -175,6 +175,32 @@ void NavierStokes::step(int nsteps, bool use_graph, hipStream_t s) {
   (void)hipEventDestroy(ev);
 }
 
+void NavierStokes::autotune(hipStream_t s) {
+  if (graph_exec_) {  // the captured step holds the old launch shapes
+    hip_check(hipStreamSynchronize(graph_stream_), "hipStreamSynchronize");
+    (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+  if (graph_) {
+    (void)hipGraphDestroy(graph_);
+    graph_ = nullptr;
+  }
+  // scratch only: yc_ and bin_ / nl_ / pb_ / pf_ are rewritten by every stage.
+  // Plan::autotune(in, out) takes the plan's input (physical, real) first and
+  // its output (spectral, complex) second, whatever direction the solver uses
+  // the plan in: each buffer must hold the side it is passed as.
+  if (form_ == 0) {
+    plan_b_->autotune(pb_.data(), yc_.data(), s);  // 3 fields: pb_ holds >= 3 physical, yc_ 3 spectral
+    plan_f_->autotune(pf_.data(), nl_.data(), s);  // 6 fields: pf_ physical, nl_ spectral
+    if (plan_f3_) plan_f3_->autotune(pb_.data(), yc_.data(), s);
+  } else {
+    plan_b_->autotune(pb_.data(), bin_.data(), s);  // 12 fields: pb_ 12 physical, bin_ 12 spectral
+    plan_f_->autotune(pf_.data(), nl_.data(), s);   // 3 fields
+    if (plan_b3_) plan_b3_->autotune(pb_.data(), yc_.data(), s);
+  }
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
 void NavierStokes::set_physical(void* const* u3, hipStream_t s) {
   Plan* pf = plan_f3_ ? plan_f3_.get() : plan_f_.get();
   pf->execute(u3, y_.data(), DFFT_FORWARD, s);

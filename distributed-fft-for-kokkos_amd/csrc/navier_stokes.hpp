@@ -30,6 +30,7 @@ class NavierStokes {
   NavierStokes& operator=(const NavierStokes&) = delete;
 
   void init_taylor_green(double v0, hipStream_t s);
+  void autotune(hipStream_t s);  // tunes every plan on scratch buffers; keeps the state
   void step(int nsteps, bool use_graph, hipStream_t s);
   double energy(hipStream_t s);
   void get_state(void* const* dst, hipStream_t s) const;

@@ -311,6 +311,7 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   p.scale = scale;
   p.herm_weight = s.kind == kC2C ? 0.0 : adj_herm_;
   p.paired = s.kind == kC2C ? 0 : s.paired;
+  p.xcd_remap = s.xcd;
   p.tw = twiddles(s.n);
   p.lines_per_block = s.lines_per_block;
   p.values_per_thread = s.values_per_thread;
@@ -348,7 +349,7 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
     double elems = 1.0;
     for (std::size_t a = 0; a < rank_; ++a) elems *= static_cast<double>(in.ext[a]);
     const double es = (s.kind == kR2C ? 0.5 : 1.0) * (fp64_ ? 16.0 : 8.0);
-    p.nontemporal = elems * es >= 256.0 * 1024 * 1024 ? 1 : 0;
+    p.nontemporal = elems * es >= 1024.0 * 1024 * 1024 ? 1 : 0;
   }
   return p;
 }
@@ -524,6 +525,7 @@ void Plan::set_option(const std::string& key, long value) {
   else if (field == "lf") passes[i].lines_fast = static_cast<int>(value);
   else if (field == "nt") passes[i].nontemporal = static_cast<int>(value);
   else if (field == "pair") passes[i].paired = static_cast<int>(value);
+  else if (field == "xcd") passes[i].xcd = static_cast<int>(value);
   else throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -541,6 +543,7 @@ long Plan::get_option(const std::string& key) const {
   if (field == "lf") return passes[i].lines_fast;
   if (field == "nt") return passes[i].nontemporal;
   if (field == "pair") return passes[i].paired;
+  if (field == "xcd") return passes[i].xcd;
   throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -564,8 +567,32 @@ float Plan::time_direction(void* const* in, void* const* out, bool forward, hipS
 // lines per block x lane order) is timed on the pass's own descriptors; then
 // the chunk size of the first two passes.  Shapes the launcher rejects
 // (too many threads or too much LDS) are skipped before launch.
+// The allocation holding p must extend at least `bytes` past p (checked when
+// the runtime knows the allocation; memory it does not track is let through).
+static void check_extent(const void* p, std::size_t bytes, const char* what) {
+  if (p == nullptr || bytes == 0) return;
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  const char* b = static_cast<const char*>(base);
+  const char* q = static_cast<const char*>(p);
+  if (q + bytes > b + size)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, std::string(what) + ": buffer smaller than the local array it must hold");
+}
+
 void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
   if (d_.exec_mode != DFFT_EXEC_FUSED) return;
+  {
+    // the tuner writes both sides: refuse arrays that cannot hold them
+    const std::size_t cs = fp64_ ? 16 : 8, rs = real_ ? cs / 2 : cs;
+    for (std::size_t r = 0; r < P_; ++r) {
+      check_extent(in[r], total_size(in_local_[r]) * rs, "autotune input");
+      check_extent(out[r], total_size(out_local_[r]) * cs, "autotune output");
+    }
+  }
   check_hip(hipSetDevice(device_), "hipSetDevice");
   hipEvent_t e0, e1;
   check_hip(hipEventCreate(&e0), "hipEventCreate");
@@ -588,13 +615,14 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
       PassSpec best_spec = passes[i];
       for (int vpt : vpts) {
         for (int lpb : lpbs) {
-          for (int lfnt = 0; lfnt < (passes[i].kind == kC2C ? 4 : 8); ++lfnt) {
+          for (int lfnt = 0; lfnt < (passes[i].kind == kC2C ? 8 : 16); ++lfnt) {
             PassSpec s = passes[i];
             s.values_per_thread = vpt;
             s.lines_per_block = lpb;
             s.lines_fast = lfnt & 1;
             s.nontemporal = (lfnt >> 1) & 1;
-            s.paired = lfnt >> 2;
+            s.xcd = (lfnt >> 2) & 1;
+            s.paired = lfnt >> 3;
             LinePass p = make_pass(s, a, b, !forward, 1.0);
             if (launch_line_pass(p, fp64_, stream) != hipSuccess) {
               (void)hipGetLastError();

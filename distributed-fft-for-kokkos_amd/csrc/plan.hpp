@@ -23,8 +23,9 @@ struct PassSpec {
   int lines_per_block = 0;
   int values_per_thread = 0;
   int lines_fast = -1;
-  int nontemporal = -1;  // -1: on for arrays of at least 256 MiB
+  int nontemporal = -1;  // -1: on for arrays of at least 1 GiB
   int paired = 1;        // R2C / C2R: two real lines per complex FFT
+  int xcd = 0;           // XCD-contiguous block numbering
 };
 
 // device table exp(-2 pi i e / n), e in [0, n): evaluated in long double,
@@ -46,9 +47,9 @@ class Plan {
   std::vector<float> execute_timed(void* const* in, void* const* out, int direction, hipStream_t stream);
   int num_passes(int direction) const;
   // tuning knobs: "chunk" (planes per chunk of the first two passes, 0 = off),
-  // "fwd.<i>.lpb|vpt|lf|nt|pair", "bwd.<i>.…" (lines per block, values per
-  // thread, lanes-across-lines, nontemporal accesses, two real lines per
-  // complex FFT in an R2C / C2R pass) of pass i
+  // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd", "bwd.<i>.…" (lines per block, values
+  // per thread, lanes-across-lines, nontemporal accesses, two real lines per
+  // complex FFT in an R2C / C2R pass, XCD-contiguous blocks) of pass i
   void set_option(const std::string& key, long value);
   long get_option(const std::string& key) const;
   // measure candidate launch shapes and chunk sizes on the caller's arrays

@@ -165,7 +165,8 @@ EXPORTED = [
     "dfft_group_execute", "dfft_group_num_launches", "dfft_group_set_option", "dfft_group_get_option",
     "dfft_group_autotune", "dfft_ns_create", "dfft_ns_destroy",
     "dfft_ns_local_extents", "dfft_ns_init_taylor_green", "dfft_ns_set_physical", "dfft_ns_set_state",
-    "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_energy", "dfft_ns_velocity", "dfft_ns_vorticity",
+    "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_autotune", "dfft_ns_energy", "dfft_ns_velocity",
+    "dfft_ns_vorticity",
     "dfft_ns_energy_spectrum", "dfft_ns_diag", "dfft_ns_rhs", "dfft_ns_rhs_tangent",
     "dfft_ns_rhs_adjoint", "dfft_ns_gradient",
 ]
@@ -234,6 +235,7 @@ def load_library(path: os.PathLike | None = None):
         "dfft_ns_set_state": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_get_state": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_step": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_autotune": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_energy": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_velocity": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_vorticity": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
@@ -497,7 +499,7 @@ class Plan:
         out = {"fwd.chunk": self.get_option("fwd.chunk"), "bwd.chunk": self.get_option("bwd.chunk")}
         for d, dirn in (("fwd", Direction.forward), ("bwd", Direction.backward)):
             for i in range(max(0, self.num_passes(dirn))):
-                for f in ("lpb", "vpt", "lf", "nt"):
+                for f in ("lpb", "vpt", "lf", "nt", "pair", "xcd"):
                     out[f"{d}.{i}.{f}"] = self.get_option(f"{d}.{i}.{f}")
         return out
 

@@ -85,6 +85,10 @@ class NavierStokes:
     def step(self, nsteps=1, use_graph=False, stream=None):
         _check(self._lib.dfft_ns_step(self._h, int(nsteps), int(bool(use_graph)), _stream_handle(stream)))
 
+    def autotune(self, stream=None):
+        """Tune the launch shapes of every plan of the solver (the state is kept)."""
+        _check(self._lib.dfft_ns_autotune(self._h, _stream_handle(stream)))
+
     def energy(self, stream=None) -> float:
         e = ctypes.c_double()
         _check(self._lib.dfft_ns_energy(self._h, ctypes.byref(e), _stream_handle(stream)))

@@ -144,9 +144,11 @@ int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int d
 /* Launch-shape knobs of a fused plan (no counterpart in the reference, whose
  * vendor FFT plans choose their own):  "chunk" / "fwd.chunk" / "bwd.chunk" =
  * planes per chunk of the first two passes (0 = off); "fwd.<i>.lpb",
- * "fwd.<i>.vpt", "fwd.<i>.lf", "fwd.<i>.nt" (and "bwd.…") = lines per block,
- * values per thread (4, 8, 16), lane order and nontemporal global accesses
- * (-1 = on for arrays of at least 256 MiB) of pass i. */
+ * "fwd.<i>.vpt", "fwd.<i>.lf", "fwd.<i>.nt", "fwd.<i>.pair", "fwd.<i>.xcd"
+ * (and "bwd.…") = lines per block, values per thread (4, 8, 16), lane order,
+ * nontemporal global accesses (-1 = on for arrays of at least 1 GiB), two
+ * real lines per complex FFT in an R2C / C2R pass (default on) and
+ * XCD-contiguous block numbering of pass i. */
 int dfft_plan_set_option(dfft_plan* plan, const char* key, long value);
 int dfft_plan_get_option(const dfft_plan* plan, const char* key, long* value);
 /* Autotuner: times every launch shape of every pass and the chunk sizes on the
@@ -227,6 +229,10 @@ int dfft_ns_get_state(dfft_ns* ns, void* const* y3, void* stream);
 /* RK4 steps (NavierStokes::solve, main.cpp:544-561); use_graph replays one
  * captured step as a hipGraph after an eager first step */
 int dfft_ns_step(dfft_ns* ns, int nsteps, int use_graph, void* stream);
+/* time the launch shapes of every plan the solver uses (on its scratch
+ * buffers; the state is kept) and keep the fastest -- no counterpart in the
+ * reference, whose plans are created once with vendor defaults */
+int dfft_ns_autotune(dfft_ns* ns, void* stream);
 /* kinetic energy, FP64 reduction (diag_energy, main.cpp:855-910) */
 int dfft_ns_energy(dfft_ns* ns, double* energy, void* stream);
 /* diagnostics (diag_fields / diag_energy, main.cpp:812-935): physical

@@ -100,6 +100,21 @@ def test_graph_replay_matches_eager_bitwise(dev):
     assert a.energy() == b.energy()
 
 
+def test_autotune_keeps_state_and_results(dev):
+    a = _solver(32, 2, 2, 0)
+    b = _solver(32, 2, 2, 0)
+    for s in (a, b):
+        s.init_taylor_green(1.0)
+    b.autotune()
+    ya0, yb0 = a.get_state(), b.get_state()
+    for ta, tb in zip(ya0, yb0):
+        assert bool((ta == tb).all())
+    a.step(3, use_graph=True)
+    b.step(3, use_graph=True)
+    ga, gb = _gather(a, a.get_state(), 1), _gather(b, b.get_state(), 1)
+    assert _rel(gb, ga) < 1e-13
+
+
 def test_rhs_of_taylor_green_matches_closed_form(dev):
     n, nu = 32, 0.01
     s = _solver(n, 2, 2, 0, nu)
