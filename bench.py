@@ -40,6 +40,9 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 N = 512
 P = 4
+# measured with rocprofv3 --pmc on this workload (profiles/fft_line_fast_512cubed_c2c_slab4.md)
+TRAFFIC_BYTES_PER_LAUNCH = 4.2956e9
+TRAFFIC_SOURCE = "profiles/fft_line_fast_512cubed_c2c_slab4.md: FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per launch"
 
 
 def log(msg):
@@ -336,8 +339,11 @@ def main():
     pass_bytes = 2 * N ** 3 * esz  # each pass reads and writes the whole array
     achieved = pass_bytes / (sum(allp) / len(allp)) / 1e6  # GB/s, average pass
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "fft_line_fast<double,512> (all three axis passes averaged)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                # HBM bytes per launch: FETCH_SIZE x2 + WRITE_SIZE x1 (copy-calibrated), rocprofv3 --pmc
+                "traffic": TRAFFIC_BYTES_PER_LAUNCH, "traffic_source": TRAFFIC_SOURCE,
+                "algorithmic_bytes_per_launch": pass_bytes,
+                "kernel": "fft_line_fast<double,512,R,NT> (the six axis passes of the round trip averaged)",
                 "dominant_pass_ms": round(dom, 4)}
     line = {
         "metric": "3D FFT fwd+bwd ms & %HBM roofline @512^3 FP64 vs FP32; NS adjoint steps/s",
