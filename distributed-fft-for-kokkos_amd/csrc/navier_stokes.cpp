@@ -112,11 +112,12 @@ std::vector<void*> NavierStokes::alloc_set(bool spectral, int nf) {
   return out;
 }
 
-void NavierStokes::rk_stage(int stage, hipStream_t s) {
+void NavierStokes::rk_stage(int stage, hipStream_t s, void* const* u_keep) {
   if (form_ == 0) {
-    plan_b_->execute(y_.data(), pb_.data(), DFFT_BACKWARD, s);
+    void* const* u = u_keep ? u_keep : pb_.data();
+    plan_b_->execute(y_.data(), u, DFFT_BACKWARD, s);
     for (int r = 0; r < P_; ++r)
-      hip_check(ns_products(static_cast<double*>(pb_[r]), static_cast<double*>(pf_[r]), phys_pts(r), s),
+      hip_check(ns_products(static_cast<const double*>(u[r]), static_cast<double*>(pf_[r]), phys_pts(r), s),
                 "ns_products");
   } else {
     for (int r = 0; r < P_; ++r)
@@ -151,7 +152,7 @@ void NavierStokes::step(int nsteps, bool use_graph, hipStream_t s) {
   if (!graph_exec_) {
     one_step(s);  // eager first step: one-time kernel attribute setup happens outside capture
     --nsteps;
-    hip_check(hipStreamCreate(&graph_stream_), "hipStreamCreate");
+    if (!graph_stream_) hip_check(hipStreamCreate(&graph_stream_), "hipStreamCreate");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     hip_check(hipStreamBeginCapture(graph_stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {

@@ -56,7 +56,21 @@ class NavierStokes {
   void apply_rhs_adjoint(void* const* y3, void* const* l3, void* const* out3, hipStream_t s);  // R'(y)^* l
 
  private:
-  void rk_stage(int stage, hipStream_t s);
+  // one RK4 stage; u_keep (optional, 3 physical fields per rank) receives the
+  // stage's physical velocity IFFT(y) instead of the scratch buffer
+  void rk_stage(int stage, hipStream_t s, void* const* u_keep = nullptr);
+  // R'(y)^* l (divergence form); u3 = IFFT(D y) if already known (else from
+  // y3); finish: 0 writes t to out3, 1/2 fuse the reverse RK4 combination
+  struct AdjFinish {
+    int mode = 0;
+    void* const* out3 = nullptr;  // mode 0
+    void* const* acc = nullptr;   // modes 1, 2
+    void* const* lam = nullptr;
+    void* const* fb = nullptr;  // mode 1: written (may alias l3)
+    double a = 0.0, c = 0.0;
+  };
+  void adjoint_core(const void* const* y3, const void* const* l3, const void* const* u3, const AdjFinish& f,
+                    hipStream_t s);
   void one_step(hipStream_t s);
   int64_t spec_pts(int r) const;
   int64_t phys_pts(int r) const;

@@ -77,6 +77,14 @@ void NavierStokes::apply_rhs_tangent(void* const* y3, void* const* d3, void* con
 }
 
 void NavierStokes::apply_rhs_adjoint(void* const* y3, void* const* l3, void* const* out3, hipStream_t s) {
+  AdjFinish f;
+  f.mode = 0;
+  f.out3 = out3;
+  adjoint_core(y3, l3, nullptr, f, s);
+}
+
+void NavierStokes::adjoint_core(const void* const* y3, const void* const* l3, const void* const* u3,
+                                const AdjFinish& f, hipStream_t s) {
   if (form_ != 0) throw Error(DFFT_ERROR_UNSUPPORTED, "adjoint needs the divergence form");
   ensure_aux();
   std::vector<void*>& mu = aux_s3a_;
@@ -87,25 +95,47 @@ void NavierStokes::apply_rhs_adjoint(void* const* y3, void* const* l3, void* con
                                  static_cast<double2*>(nl_[r]), s),
               "adjoint_prepare");
   plan_f_->execute(nl_.data(), pf_.data(), DFFT_BACKWARD, s);  // S_mj (6 fields)
-  for (int r = 0; r < P_; ++r)  // D y into ch, which is free until the forward transform of c
-    hip_check(ns_dealias(g_, sblk_[r], static_cast<const double2*>(y3[r]), static_cast<double2*>(ch[r]), s),
-              "dealias");
-  plan_b_->execute(ch.data(), pb_.data(), DFFT_BACKWARD, s);  // u = IFFT(D y)
+  if (!u3) {
+    for (int r = 0; r < P_; ++r)  // D y into ch, which is free until the forward transform of c
+      hip_check(ns_dealias(g_, sblk_[r], static_cast<const double2*>(y3[r]), static_cast<double2*>(ch[r]), s),
+                "dealias");
+    plan_b_->execute(ch.data(), pb_.data(), DFFT_BACKWARD, s);  // u = IFFT(D y)
+    u3 = pb_.data();
+  }
   for (int r = 0; r < P_; ++r)
-    hip_check(ns_adjoint_contract(static_cast<double*>(pb_[r]), static_cast<double*>(pf_[r]),
+    hip_check(ns_adjoint_contract(static_cast<const double*>(u3[r]), static_cast<double*>(pf_[r]),
                                   static_cast<double*>(c[r]), phys_pts(r), s),
               "adjoint_contract");
   plan_f3_->execute(c.data(), ch.data(), DFFT_FORWARD, s);
-  for (int r = 0; r < P_; ++r)
-    hip_check(ns_adjoint_finish(g_, sblk_[r], static_cast<const double2*>(ch[r]), static_cast<const double2*>(mu[r]),
-                                static_cast<double2*>(out3[r]), s),
-              "adjoint_finish");
+  for (int r = 0; r < P_; ++r) {
+    const auto* chr = static_cast<const double2*>(ch[r]);
+    const auto* mur = static_cast<const double2*>(mu[r]);
+    if (f.mode == 0)
+      hip_check(ns_adjoint_finish(g_, sblk_[r], chr, mur, static_cast<double2*>(f.out3[r]), s), "adjoint_finish");
+    else
+      hip_check(ns_adjoint_finish_rk(g_, sblk_[r], chr, mur, static_cast<double2*>(f.acc[r]),
+                                     static_cast<double2*>(f.lam[r]), f.fb ? static_cast<double2*>(f.fb[r]) : nullptr,
+                                     f.a, f.c, f.mode, s),
+                "adjoint_finish_rk");
+  }
 }
 
 // J(u0) = kinetic energy after `nsteps` RK4 steps from x0 = D P FFT(u0);
 // writes dJ/du0 (plain inner product on the physical grid) to `grad`.
-// Checkpoints every `ck` steps; each segment is recomputed once, keeping its
-// RK stage values, and then reversed.
+// Checkpoints every `ck` steps.  A segment's forward recomputation keeps the
+// physical velocity u = IFFT(y) of every RK stage (the only state the
+// adjoint right-hand side needs); the last segment keeps them during the
+// first forward sweep and is not recomputed.
+//
+// Reverse step (x the step input, x_k the stage inputs, yc = D x):
+//   k_k = dt R(x_k), x_1 = yc + k_1/2, x_2 = yc + k_2/2, x_3 = yc + k_3,
+//   x_new = yc + (k_1 + 2 k_2 + 2 k_3 + k_4)/6
+// gives, with t_k = R'(x_k)^* of the stage seed fb_k:
+//   fb_3 = dt lam/6,              t_3 = R'(x_3)^* fb_3
+//   fb_2 = dt (lam/3 + t_3),      t_2 = R'(x_2)^* fb_2
+//   fb_1 = dt (lam/3 + t_2/2),    t_1 = R'(x_1)^* fb_1
+//   fb_0 = dt (lam/6 + t_1/2),    t_0 = R'(x_0)^* fb_0
+//   lam_new = D (lam + t_3 + t_2 + t_1 + t_0).
 double NavierStokes::gradient(int nsteps, int ck, void* const* u0, void* const* grad, hipStream_t s) {
   if (form_ != 0) throw Error(DFFT_ERROR_UNSUPPORTED, "gradient needs the divergence form");
   if (nsteps < 0 || ck < 1) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "bad step counts");
@@ -120,72 +150,60 @@ double NavierStokes::gradient(int nsteps, int ck, void* const* u0, void* const* 
       if (spec_pts(r) > 0)
         hip_check(hipMemcpyAsync(dst[r], src[r], size_t(3 * spec_pts(r)) * 16, hipMemcpyDeviceToDevice, s), "copy");
   };
-  auto axpby = [&](double a, const std::vector<void*>& x, double b, const std::vector<void*>& y) {
-    for (int r = 0; r < P_; ++r)
-      hip_check(ns_axpby(3 * spec_pts(r), a, static_cast<double2*>(x[r]), b, static_cast<const double2*>(y[r]), s),
-                "axpby");
-  };
   double J = 0.0;
   try {
     set_physical(u0, s);
     const int nseg = (nsteps + ck - 1) / ck;
+    const int seg_len = std::min(ck, std::max(nsteps, 1));
     std::vector<std::vector<void*>> ckpt;
     for (int k = 0; k < nseg; ++k) ckpt.push_back(fresh(true, 3));
+    std::vector<std::vector<void*>> stage_u;  // 3 physical fields per stage of a segment
+    for (int k = 0; k < 4 * seg_len; ++k) stage_u.push_back(fresh(false, 3));
+    const int last_first = nseg > 0 ? (nseg - 1) * ck : 0;
     for (int st = 0; st < nsteps; ++st) {
       if (st % ck == 0) copy3(ckpt[st / ck], y_);
-      one_step(s);
+      if (st >= last_first) {
+        for (int stage = 0; stage < 4; ++stage) rk_stage(stage, s, stage_u[4 * (st - last_first) + stage].data());
+      } else {
+        one_step(s);
+      }
     }
     J = energy(s);
-    std::vector<void*> lam = fresh(true, 3);
+    std::vector<void*> lam = fresh(true, 3), acc = fresh(true, 3), fb = fresh(true, 3);
     copy3(lam, y_);
-    axpby(g_.coef, lam, 0.0, lam);
-    std::vector<std::vector<void*>> stage_states;
-    for (int k = 0; k < 4 * std::min(ck, std::max(nsteps, 1)); ++k) stage_states.push_back(fresh(true, 3));
-    std::vector<void*> fb[4] = {fresh(true, 3), fresh(true, 3), fresh(true, 3), fresh(true, 3)};
-    std::vector<void*> yb = fresh(true, 3), t = fresh(true, 3);
+    for (int r = 0; r < P_; ++r)
+      hip_check(ns_axpby(3 * spec_pts(r), g_.coef, static_cast<double2*>(lam[r]), 0.0,
+                         static_cast<const double2*>(lam[r]), s),
+                "axpby");
+    AdjFinish f;
+    f.acc = acc.data();
+    f.lam = lam.data();
+    f.fb = fb.data();
     for (int seg = nseg - 1; seg >= 0; --seg) {
       const int first = seg * ck, last = std::min(nsteps, first + ck);
-      copy3(y_, ckpt[seg]);
-      for (int st = first; st < last; ++st) {
-        for (int stage = 0; stage < 4; ++stage) {
-          copy3(stage_states[4 * (st - first) + stage], y_);
-          rk_stage(stage, s);
-        }
+      if (seg != nseg - 1) {
+        copy3(y_, ckpt[seg]);
+        for (int st = first; st < last; ++st)
+          for (int stage = 0; stage < 4; ++stage) rk_stage(stage, s, stage_u[4 * (st - first) + stage].data());
       }
       for (int st = last - 1; st >= first; --st) {
-        auto& y0 = stage_states[4 * (st - first) + 0];
-        auto& y1 = stage_states[4 * (st - first) + 1];
-        auto& y2 = stage_states[4 * (st - first) + 2];
-        auto& y3 = stage_states[4 * (st - first) + 3];
-        // seed: fb0 = dt/6 l, fb1 = dt/3 l, fb2 = dt/3 l, fb3 = dt/6 l, yb = l
-        copy3(fb[0], lam);
-        axpby(dt / 6.0, fb[0], 0.0, lam);
-        copy3(fb[1], lam);
-        axpby(dt / 3.0, fb[1], 0.0, lam);
-        copy3(fb[2], lam);
-        axpby(dt / 3.0, fb[2], 0.0, lam);
-        copy3(fb[3], lam);
-        axpby(dt / 6.0, fb[3], 0.0, lam);
-        copy3(yb, lam);
-        apply_rhs_adjoint(y3.data(), fb[3].data(), t.data(), s);
-        axpby(1.0, yb, 1.0, t);
-        axpby(1.0, fb[2], dt, t);
-        apply_rhs_adjoint(y2.data(), fb[2].data(), t.data(), s);
-        axpby(1.0, yb, 1.0, t);
-        axpby(1.0, fb[1], dt / 2.0, t);
-        apply_rhs_adjoint(y1.data(), fb[1].data(), t.data(), s);
-        axpby(1.0, yb, 1.0, t);
-        axpby(1.0, fb[0], dt / 2.0, t);
-        apply_rhs_adjoint(y0.data(), fb[0].data(), t.data(), s);
-        axpby(1.0, yb, 1.0, t);
-        // the stage-0 input is dealiased in place (main.cpp:576-578)
+        const int base = 4 * (st - first);
         for (int r = 0; r < P_; ++r)
-          hip_check(ns_dealias(g_, sblk_[r], static_cast<const double2*>(yb[r]), static_cast<double2*>(yb[r]), s),
-                    "dealias");
-        copy3(lam, yb);
+          hip_check(ns_rk_adjoint_init(3 * spec_pts(r), static_cast<const double2*>(lam[r]),
+                                       static_cast<double2*>(fb[r]), static_cast<double2*>(acc[r]), dt / 6.0, s),
+                    "rk_adjoint_init");
+        const double a[3] = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0}, c[3] = {1.0, 0.5, 0.5};
+        for (int k = 3; k >= 1; --k) {
+          f.mode = 1;
+          f.a = a[3 - k];
+          f.c = c[3 - k];
+          adjoint_core(nullptr, fb.data(), stage_u[base + k].data(), f, s);
+        }
+        f.mode = 2;
+        adjoint_core(nullptr, fb.data(), stage_u[base].data(), f, s);
       }
     }
-    // x0 = D P FFT(u0)  =>  dJ/du0 = n^3 IFFT(D P l0)
+    // x0 = D P FFT(u0)  =>  dJ/du0 = n^3 IFFT(D P lam_0)
     for (int r = 0; r < P_; ++r)
       hip_check(ns_dealias_project(g_, sblk_[r], static_cast<double2*>(lam[r]), s), "dealias_project");
     Plan* pb3 = plan_b3_ ? plan_b3_.get() : plan_b_.get();

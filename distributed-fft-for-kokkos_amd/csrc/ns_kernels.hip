@@ -528,3 +528,65 @@ hipError_t ns_bin_sum(const double* partials, int rows, int nb, double* out, hip
 }
 
 }  // namespace dfft
+
+namespace dfft {
+
+// ---------------------------------------------------------------------------
+// Reverse RK4 step: the adjoint's finishing step fused with the stage
+// combination.  With t = -D chat - nu k^2 mu (= R'(y)^* of the stage seed):
+//   mode 1: acc += t;  fb = dt (a lam + c t)   (the next stage's seed)
+//   mode 2: lam = D (acc + t)                  (the adjoint of the step input)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) adjoint_finish_rk_kernel(const NsGrid g, const NsBlock b, const double2* chat,
+                                                                const double2* mu, double2* acc, double2* lam,
+                                                                double2* fb, double a, double c, int mode) {
+  const int64_t m_tot = static_cast<int64_t>(b.ext[0]) * b.ext[1] * b.ext[2];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; m < m_tot; m += stride) {
+    const Mode md = mode_of(g, b, m);
+    const double visc = -g.nu * md.ksq;
+    for (int k = 0; k < 3; ++k) {
+      const int64_t e = k * m_tot + m;
+      const double2 ch = chat[e], u = mu[e];
+      const double2 t{-md.mask * ch.x + visc * u.x, -md.mask * ch.y + visc * u.y};
+      const double2 s = acc[e];
+      const double2 s2{s.x + t.x, s.y + t.y};
+      if (mode == 1) {
+        acc[e] = s2;
+        const double2 l = lam[e];
+        fb[e] = double2{g.dt * (a * l.x + c * t.x), g.dt * (a * l.y + c * t.y)};
+      } else {
+        lam[e] = cscale(s2, md.mask);
+      }
+    }
+  }
+}
+
+hipError_t ns_adjoint_finish_rk(const NsGrid& g, const NsBlock& b, const double2* chat3, const double2* mu3,
+                                double2* acc3, double2* lam3, double2* fb3, double a, double c, int mode,
+                                hipStream_t s) {
+  const int64_t m = static_cast<int64_t>(b.ext[0]) * b.ext[1] * b.ext[2];
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(adjoint_finish_rk_kernel, dim3(blocks_for(m)), dim3(256), 0, s, g, b, chat3, mu3, acc3, lam3,
+                     fb3, a, c, mode);
+  return hipGetLastError();
+}
+
+// fb = c lam, acc = lam (start of a reverse RK4 step)
+__global__ void __launch_bounds__(256) rk_adjoint_init_kernel(int64_t n, const double2* lam, double2* fb,
+                                                              double2* acc, double c) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double2 l = lam[i];
+    fb[i] = double2{c * l.x, c * l.y};
+    acc[i] = l;
+  }
+}
+
+hipError_t ns_rk_adjoint_init(int64_t n, const double2* lam, double2* fb, double2* acc, double c, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(rk_adjoint_init_kernel, dim3(blocks_for(n)), dim3(256), 0, s, n, lam, fb, acc, c);
+  return hipGetLastError();
+}
+
+}  // namespace dfft

@@ -82,6 +82,14 @@ hipError_t ns_dealias(const NsGrid& g, const NsBlock& b, const double2* src3, do
 // x *= a over n doubles
 hipError_t ns_scale(int64_t n, double a, double* x, hipStream_t s);
 
+// reverse RK4 step pieces: t = -D chat - nu k^2 mu; mode 1: acc += t,
+// fb = dt (a lam + c t); mode 2: lam = D (acc + t)
+hipError_t ns_adjoint_finish_rk(const NsGrid& g, const NsBlock& b, const double2* chat3, const double2* mu3,
+                                double2* acc3, double2* lam3, double2* fb3, double a, double c, int mode,
+                                hipStream_t s);
+// fb = c lam, acc = lam over n complex values
+hipError_t ns_rk_adjoint_init(int64_t n, const double2* lam, double2* fb, double2* acc, double c, hipStream_t s);
+
 // diagnostics (main.cpp:781-910)
 constexpr int kMaxSpectrumBins = 256;
 // w = curl of y: i (ky w - kz v), i (kz u - kx w), i (kx v - ky u)
