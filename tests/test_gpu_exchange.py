@@ -128,3 +128,23 @@ def test_exchange_matches_pack_all2all_unpack(dev, case):
         torch.cuda.synchronize()
         for r in range(plan.P):
             assert np.array_equal(outs[r].cpu().numpy(), ref[r]), (case, bi, r)
+
+
+@pytest.mark.parametrize("shape", [[8, 7, 5], [4, 6, 3, 5]])
+def test_local_transpose_is_numpy_transpose(dev, shape):
+    """kokkos-fft Impl::transpose(in, out, map) on one rank (SlabPlan.hpp final
+    local transposes): out.extent(k) = in.extent(map[k]), out = numpy.transpose(in, map),
+    bit for bit, through dfft_exchange with a single rank."""
+    import itertools
+    import torch
+    import dfft
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    ones = [1] * len(shape)
+    ident = list(range(len(shape)))
+    for m in itertools.permutations(ident):
+        want = np.transpose(x, m)
+        out = torch.zeros(list(want.shape), dtype=torch.complex128, device=dev)
+        dfft.exchange([torch.from_numpy(x).to(dev)], [out], shape, ones, ident, ones, list(m))
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), want), m
