@@ -340,15 +340,20 @@ void Group::autotune(void* const* in, void* const* out, hipStream_t s) {
         set_option("nt", nt);
         execute(in, out, DFFT_FORWARD, false, s);  // builds the tables (warm-up)
         execute(out, in, DFFT_BACKWARD, false, s);
-        check(hipEventRecord(e0, s), "hipEventRecord");
-        for (int rep = 0; rep < 3; ++rep) {
-          execute(in, out, DFFT_FORWARD, false, s);
-          execute(out, in, DFFT_BACKWARD, false, s);
+        // best of two trials of three round trips: single trials are noisy
+        float ms = -1.0f;
+        for (int trial = 0; trial < 2; ++trial) {
+          check(hipEventRecord(e0, s), "hipEventRecord");
+          for (int rep = 0; rep < 3; ++rep) {
+            execute(in, out, DFFT_FORWARD, false, s);
+            execute(out, in, DFFT_BACKWARD, false, s);
+          }
+          check(hipEventRecord(e1, s), "hipEventRecord");
+          check(hipEventSynchronize(e1), "hipEventSynchronize");
+          float t = 0.0f;
+          check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+          if (ms < 0.0f || t < ms) ms = t;
         }
-        check(hipEventRecord(e1, s), "hipEventRecord");
-        check(hipEventSynchronize(e1), "hipEventSynchronize");
-        float ms = 0.0f;
-        check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
         if (best < 0.0f || ms < best) {
           best = ms;
           best_cap = cap;

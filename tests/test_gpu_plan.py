@@ -165,3 +165,14 @@ def test_nonsmooth_lengths_generic_kernel(dev):
     for gin in ([11, 13, 17], [9, 25, 49], [96, 10, 160]):
         for real in [False, True]:
             _check(dev, gin, [0, 1, 2], [2, 1, 1], [1, 2, 1], real, seed=9)
+
+
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("fp64", [True, False])
+def test_mixed_radix_fast_lengths(dev, real, fp64):
+    """Compile-time kernels for 96 = 4*4*2*3 and 160 = 4*4*2*5 (several butterflies per
+    thread in the first stage) along the contiguous and the strided axes."""
+    for gin, axes in (([6, 5, 96], [2]), ([4, 3, 160], [2]), ([96, 6, 5], [0]), ([160, 4, 3], [0]),
+                      ([8, 6, 160], [0, 1, 2]), ([96, 4, 6], [1, 2, 0])):
+        _check(dev, gin, axes, [1, 1, 1], [1, 1, 1], real, fp64, seed=5)
+        _check(dev, gin, axes, [1, 2, 1], [1, 2, 1], real, fp64, seed=6)
