@@ -64,6 +64,9 @@ struct DDesc {
   int32_t map[kMaxDim];
   FastDiv div_big[kMaxDim];    // base + 1
   FastDiv div_small[kMaxDim];  // base
+  // > 0: row pitch (elements) of the innermost storage axis of a LayoutRight
+  // single-rank array, padded so rows start on 128-byte boundaries
+  int32_t pitch;
   char* ptr[kMaxRanks];
 };
 
@@ -113,7 +116,7 @@ __device__ __forceinline__ char* element_ptr(const DDesc& d, const int32_t* idx,
         int a = d.map[k];
         off += lidx[a] * acc;
         if (a == sa) stride_sa = acc;
-        acc *= llen[a];
+        acc *= (k == d.ndim - 1 && d.pitch > 0) ? d.pitch : llen[a];
       }
     }
   } else {

@@ -288,6 +288,15 @@ __device__ __forceinline__ void decode_line(const LinePass& p, int32_t line, int
   }
 }
 
+// true if line `line` of a pass enumerated over a padded extent is padding
+template <typename Real>
+__device__ __forceinline__ bool padding_line(const LinePass& p, int32_t line) {
+  if (p.pad_real <= 0) return false;
+  int32_t idx[kMaxDim];
+  decode_line<Real>(p, line, idx);
+  return idx[p.pad_axis] >= p.pad_real;
+}
+
 template <typename Real, bool NT = false>
 __device__ __forceinline__ LineIO<Real, NT> make_line_io(const LinePass& p, int32_t line) {
   using C = cplx_t<Real>;
@@ -459,7 +468,7 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
     t = 0;
   }
   const int32_t line = blk * B + b;
-  const bool valid = active && line < p.nlines;
+  const bool valid = active && line < p.nlines && !padding_line<Real>(p, line);
   const double2* tw = reinterpret_cast<const double2*>(p.tw);
   C* line_lds = lds + b * Cfg::LDSN;
 
@@ -522,7 +531,7 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
     t = 0;
   }
   const int32_t pair = blk * B + b;
-  const bool valid = active && 2 * pair < p.nlines;
+  const bool valid = active && 2 * pair < p.nlines;  // paired passes never enumerate padding
   const double2* tw = reinterpret_cast<const double2*>(p.tw);
   C* line_lds = lds + b * Cfg::LDSN;
 
@@ -665,7 +674,7 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
     const int32_t b = e / n, pos = e - b * n;
     const int32_t line = blockIdx.x * B + b;
     C v{Real(0), Real(0)};
-    if (line < p.nlines) {
+    if (line < p.nlines && !padding_line<Real>(p, line)) {
       int32_t idx[kMaxDim];
       decode_line<Real>(p, line, idx);
       LineIO<Real> io;
@@ -733,7 +742,7 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
   for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
     const int32_t b = e / n, pos = e - b * n;
     const int32_t line = blockIdx.x * B + b;
-    if (line >= p.nlines) continue;
+    if (line >= p.nlines || padding_line<Real>(p, line)) continue;
     if (p.kind == kR2C && 2 * pos > n) continue;
     int32_t idx[kMaxDim];
     decode_line<Real>(p, line, idx);

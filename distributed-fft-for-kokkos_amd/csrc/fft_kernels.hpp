@@ -34,6 +34,11 @@ struct LinePass {
   // 1: an R2C (forward) or C2R (inverse) pass on the fast kernel transforms
   // two real lines per complex FFT (lines 2q and 2q+1 of the pass)
   int32_t paired;
+  // pad_real > 0: batch axis pad_axis is enumerated over a padded extent
+  // (batch_ext, the row pitch of a padded array) and lines whose index along
+  // it is >= pad_real are skipped, so blocks of lines start on aligned rows
+  int32_t pad_axis;
+  int32_t pad_real;
   // 1: blocks are renumbered so each of the 8 XCDs (which take workgroups
   // round-robin) walks one contiguous range of lines
   int32_t xcd_remap;

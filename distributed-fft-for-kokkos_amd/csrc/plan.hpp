@@ -47,6 +47,7 @@ class Plan {
   std::vector<float> execute_timed(void* const* in, void* const* out, int direction, hipStream_t stream);
   int num_passes(int direction) const;
   // tuning knobs: "chunk" (planes per chunk of the first two passes, 0 = off),
+  // "pad" (scratch rows padded to 128-byte multiples: -1 auto, 0 off, 1 on),
   // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd", "bwd.<i>.…" (lines per block, values
   // per thread, lanes-across-lines, nontemporal accesses, two real lines per
   // complex FFT in an R2C / C2R pass, XCD-contiguous blocks) of pass i
@@ -67,7 +68,10 @@ class Plan {
 
  private:
   DDesc user_desc(const Vec& ext, const Vec& topo, Layout tl, void* const* ptrs) const;
-  DDesc scratch_desc(const Vec& ext, void* p) const;
+  // the plan scratch holding an array of extents `ext` with elements of
+  // `elem_bytes` bytes; LayoutRight rows are padded to 128-byte multiples
+  DDesc scratch_desc(const Vec& ext, void* p, std::size_t elem_bytes) const;
+  std::size_t scratch_elems(const Vec& ext, std::size_t elem_bytes) const;  // incl. padding
   LinePass make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, bool inverse, double scale,
                      int slow_axis = -1) const;
   float time_direction(void* const* in, void* const* out, bool forward, hipStream_t s, int reps);
@@ -96,6 +100,7 @@ class Plan {
   std::map<int, void*> tw_;
   std::vector<PassSpec> fwd_, bwd_;
   int chunk_[2] = {0, 0};  // planes per chunk, forward / backward (0 = no chunking)
+  int pad_ = -1;           // scratch row padding: -1 auto (<= 1/16 extra), 0 off, 1 on
   // set only while execute_adjoint runs: scale of the adjoint direction and
   // the interior half-spectrum weight of its R2C / C2R line pass
   double adj_scale_ = -1.0;

@@ -143,7 +143,9 @@ int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int d
 
 /* Launch-shape knobs of a fused plan (no counterpart in the reference, whose
  * vendor FFT plans choose their own):  "chunk" / "fwd.chunk" / "bwd.chunk" =
- * planes per chunk of the first two passes (0 = off); "fwd.<i>.lpb",
+ * planes per chunk of the first two passes (0 = off); "pad" = rows of the
+ * plan scratch padded to 128-byte multiples (-1 auto: when that adds at most
+ * 1/16, 0 off, 1 on); "fwd.<i>.lpb",
  * "fwd.<i>.vpt", "fwd.<i>.lf", "fwd.<i>.nt", "fwd.<i>.pair", "fwd.<i>.xcd"
  * (and "bwd.…") = lines per block, values per thread (4, 8, 16), lane order,
  * nontemporal global accesses (-1 = on for arrays of at least 1 GiB), two

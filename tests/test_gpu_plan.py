@@ -176,3 +176,21 @@ def test_mixed_radix_fast_lengths(dev, real, fp64):
                       ([8, 6, 160], [0, 1, 2]), ([96, 4, 6], [1, 2, 0])):
         _check(dev, gin, axes, [1, 1, 1], [1, 1, 1], real, fp64, seed=5)
         _check(dev, gin, axes, [1, 2, 1], [1, 2, 1], real, fp64, seed=6)
+
+
+@pytest.mark.parametrize("pad", [0, 1])
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("fp64", [True, False])
+def test_padded_scratch_rows(dev, pad, real, fp64):
+    """Scratch rows padded to 128-byte multiples (forced on) or not at all, for odd row
+    lengths: slab pairs in every axis order, a pencil pair, and chunked first passes."""
+    cfg = lambda plan, ins, outs: plan.set_option("pad", pad)
+    for tin, tout in itertools.product([[1, 1, 2], [1, 2, 1], [2, 1, 1]], repeat=2):
+        for axes in itertools.permutations([0, 1, 2]):
+            _check(dev, [8, 7, 5], list(axes), tin, tout, real, fp64, seed=3, configure=cfg)
+    _check(dev, [9, 8, 13], [0, 1, 2], [2, 2, 1], [1, 2, 2], real, fp64, seed=4, configure=cfg)
+
+    def chunked(plan, ins, outs):
+        plan.set_option("pad", pad)
+        plan.set_option("chunk", 3)
+    _check(dev, [12, 10, 9], [0, 1, 2], [2, 1, 1], [1, 2, 1], real, fp64, seed=5, configure=chunked)
