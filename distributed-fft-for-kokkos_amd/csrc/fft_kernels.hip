@@ -595,17 +595,30 @@ __device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nb) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Block of the pass for hardware block b: optional XCD-contiguous numbering,
+// then an optional column-major walk of the (nb / C) x C block grid, where C
+// blocks cover one row of the fastest batch axis (p.swizzle = C; the launcher
+// sets it only when it divides the grid).
+__device__ __forceinline__ int32_t pass_block(const LinePass& p, int32_t b, int32_t nb) {
+  if (p.xcd_remap) b = xcd_block(b, nb);
+  if (p.swizzle > 1) {
+    const int32_t rows = nb / p.swizzle;
+    b = (b % rows) * p.swizzle + b / rows;
+  }
+  return b;
+}
+
 template <typename Real, int N, int RM, bool NT>
 __global__ void __launch_bounds__(1024) fft_pair_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
-  const int32_t blk = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : static_cast<int32_t>(blockIdx.x);
+  const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
   fast_pair_body<Real, N, RM, NT>(p, blk, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
 }
 
 template <typename Real, int N, int RM, bool NT>
 __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
-  const int32_t blk = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : static_cast<int32_t>(blockIdx.x);
+  const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
   fast_line_body<Real, N, RM, NT>(p, blk, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
 }
 
@@ -809,6 +822,13 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
   const int64_t units = pair ? (static_cast<int64_t>(p.nlines) + 1) / 2 : p.nlines;
   const unsigned grid = static_cast<unsigned>((units + B - 1) / B);
   if (grid == 0) return hipSuccess;
+  if (p.swizzle) {
+    // C = blocks per row of the fastest batch axis; the walk needs whole rows
+    const int32_t inner = p.nbatch > 0 ? p.batch_ext[p.nbatch - 1] : 0;
+    const int32_t per = pair ? 2 * B : B;
+    const int32_t C = (inner > 0 && inner % per == 0) ? inner / per : 0;
+    p.swizzle = (C > 1 && grid % static_cast<unsigned>(C) == 0 && grid / C > 1) ? C : 0;
+  }
   switch (v) {
     case 0: hipLaunchKernelGGL((fft_line_fast<Real, N, RM, false>), dim3(grid), dim3(T * B), smem, s, p); break;
     case 1: hipLaunchKernelGGL((fft_line_fast<Real, N, RM, true>), dim3(grid), dim3(T * B), smem, s, p); break;

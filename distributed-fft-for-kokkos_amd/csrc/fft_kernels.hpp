@@ -39,6 +39,10 @@ struct LinePass {
   // it is >= pad_real are skipped, so blocks of lines start on aligned rows
   int32_t pad_axis;
   int32_t pad_real;
+  // 1: consecutive blocks step along the slower batch axes first (the block
+  // grid, rows of blocks along the fastest batch axis, is walked column by
+  // column), spreading concurrent blocks over more of the array
+  int32_t swizzle;
   // 1: blocks are renumbered so each of the 8 XCDs (which take workgroups
   // round-robin) walks one contiguous range of lines
   int32_t xcd_remap;

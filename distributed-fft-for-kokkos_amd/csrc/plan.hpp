@@ -26,6 +26,7 @@ struct PassSpec {
   int nontemporal = -1;  // -1: on for arrays of at least 1 GiB
   int paired = 1;        // R2C / C2R: two real lines per complex FFT
   int xcd = 0;           // XCD-contiguous block numbering
+  int swz = 0;           // column-major walk of the block grid
 };
 
 // device table exp(-2 pi i e / n), e in [0, n): evaluated in long double,
@@ -48,9 +49,10 @@ class Plan {
   int num_passes(int direction) const;
   // tuning knobs: "chunk" (planes per chunk of the first two passes, 0 = off),
   // "pad" (scratch rows padded to 128-byte multiples: -1 auto, 0 off, 1 on),
-  // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd", "bwd.<i>.…" (lines per block, values
-  // per thread, lanes-across-lines, nontemporal accesses, two real lines per
-  // complex FFT in an R2C / C2R pass, XCD-contiguous blocks) of pass i
+  // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd|swz", "bwd.<i>.…" (lines per block,
+  // values per thread, lanes-across-lines, nontemporal accesses, two real
+  // lines per complex FFT in an R2C / C2R pass, XCD-contiguous blocks,
+  // column-major block walk) of pass i
   void set_option(const std::string& key, long value);
   long get_option(const std::string& key) const;
   // measure candidate launch shapes and chunk sizes on the caller's arrays
