@@ -22,7 +22,10 @@ Further sections of the same line:
 * mixed_batch: 64 C2C transforms, N cycling through {96, 128, 160}^3
   complex64, forward + backward, one grouped launch per pass, hipGraph;
 * precision: FP32 against FP64 (rel-L2) for 512^3 C2C and 1024^3 R2C on
-  U[-1,1] inputs, and against the exact spectrum of sin(k.x), k = (3, 5, 7).
+  U[-1,1] inputs, and against the exact spectrum of sin(k.x), k = (3, 5, 7);
+* rocfft_comparator: the vendor single-GPU FFT (torch.fft -> hipFFT/rocFFT)
+  on the same 512^3 round trip, for reference;
+* cpu_library_baseline: scipy.fft on all host cores beside the oracle.
 """
 
 import argparse
@@ -108,6 +111,44 @@ def cpu_baseline():
             "cores": 1, "kind": "port",
             "sample": f"oracle slab plan P=4, one {n}^3 complex128 fwd+bwd in {dt:.2f} s, "
                       f"scaled x{scale:.1f} by N log N"}
+
+
+def cpu_library_baseline():
+    """scipy.fft (pocketfft) on all host cores: a library CPU bound beside the
+    oracle (one 256^3 complex128 fftn + ifftn, scaled to 512^3 by N log N)."""
+    import numpy as np
+    import scipy.fft as sf
+    n = 256
+    cores = len(os.sched_getaffinity(0))
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, (n,) * 3) + 1j * rng.uniform(-1, 1, (n,) * 3)
+    sf.ifftn(sf.fftn(x, workers=cores), workers=cores)  # warm-up
+    t0 = time.perf_counter()
+    sf.ifftn(sf.fftn(x, workers=cores), workers=cores)
+    dt = time.perf_counter() - t0
+    scale = (N ** 3 * math.log2(N ** 3)) / (n ** 3 * math.log2(n ** 3))
+    return {"value": dt * 1e3 * scale, "unit": "ms per 512^3 complex128 fwd+bwd (extrapolated)", "cores": cores,
+            "kind": "library", "sample": f"scipy.fft fftn+ifftn {n}^3 complex128 in {dt:.3f} s, x{scale:.1f}"}
+
+
+def rocfft_comparator(torch, dev, steps):
+    """The vendor single-GPU 3-D FFT (torch.fft -> hipFFT/rocFFT) on the same
+    512^3 round trip, no distribution: a reference point, not this library."""
+    out = {}
+    for tag, dt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
+        g = torch.Generator(device=dev).manual_seed(3)
+        rdt = torch.float64 if dt == torch.complex128 else torch.float32
+        x = torch.complex(torch.rand((N,) * 3, generator=g, device=dev, dtype=rdt),
+                          torch.rand((N,) * 3, generator=g, device=dev, dtype=rdt))
+        for _ in range(2):
+            y = torch.fft.ifftn(torch.fft.fftn(x))
+        del y
+        ms = _sync_ms(torch, lambda: torch.fft.ifftn(torch.fft.fftn(x)), steps)
+        out[tag + "_ms"] = round(ms, 4)
+        del x
+        torch.cuda.empty_cache()
+    out["what"] = "torch.fft.fftn + ifftn (hipFFT/rocFFT), 512^3, one GPU array, incl. its temporaries"
+    return out
 
 
 def _sync_ms(torch, fn, reps):
@@ -272,8 +313,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the plan autotuner")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
                     help="run one precision only (profiling runs)")
-    ap.add_argument("--sections", default="fft,ns,adjoint,batch,precision",
-                    help="comma list of: fft (headline), ns, adjoint, batch, precision")
+    ap.add_argument("--sections", default="fft,ns,adjoint,batch,precision,rocfft",
+                    help="comma list of: fft (headline), ns, adjoint, batch, precision, rocfft")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -364,9 +405,14 @@ def main():
         "roofline": roofline,
     }
     run_sections(torch, dfft, dev, args, sections, line)
+    if "rocfft" in sections:
+        log("[bench] rocFFT comparator ...")
+        line["rocfft_comparator"] = rocfft_comparator(torch, dev, max(3, args.steps // 2))
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline()
+        line["cpu_library_baseline"] = cpu_library_baseline()
     print(json.dumps(line), flush=True)
 
 
