@@ -13,6 +13,8 @@
 
 #include "fft_kernels.hpp"
 
+#include <type_traits>
+
 namespace dfft {
 
 // ---------------------------------------------------------------------------
@@ -405,9 +407,65 @@ __device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const doub
   }
 }
 
+// Bytes of one line's LDS buffer: LDSN complex values, or LDSN reals when
+// the exchanges travel in two halves (SPLIT).
+template <typename Real, int N, int RM, bool SPLIT> constexpr size_t line_lds_bytes() {
+  return static_cast<size_t>(FastCfg<Real, N, RM>::LDSN) * (SPLIT ? sizeof(Real) : sizeof(cplx_t<Real>));
+}
+
+// Register <-> LDS permutation of one line: every lane stores its values
+// v[q*RW + j] at positions wpos(q, j), then loads v[q*RR + j] from positions
+// rpos(q, j) (NW*RW = NR*RR = R).  SPLIT moves the real parts first and the
+// imaginary parts second through a buffer of reals: after the first half a
+// lane's old real parts are in LDS and can be overwritten, its old imaginary
+// parts are still in registers for the second half.
+template <typename Real, int N, int RM, bool SPLIT, int NW, int RW, int NR, int RR, typename WPos, typename RPos>
+__device__ __forceinline__ void lds_permute(cplx_t<Real>* v, void* line, WPos wpos, RPos rpos) {
+  using Cfg = FastCfg<Real, N, RM>;
+  if constexpr (SPLIT) {
+    Real* lr = static_cast<Real*>(line);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+#pragma unroll
+        for (int j = 0; j < RW; ++j) lr[Cfg::lidx(wpos(q, j))] = h ? v[q * RW + j].y : v[q * RW + j].x;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+#pragma unroll
+        for (int j = 0; j < RR; ++j) {
+          const Real x = lr[Cfg::lidx(rpos(q, j))];
+          if (h) {
+            v[q * RR + j].y = x;
+          } else {
+            v[q * RR + j].x = x;
+          }
+        }
+      }
+    }
+  } else {
+    cplx_t<Real>* lc = static_cast<cplx_t<Real>*>(line);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+#pragma unroll
+      for (int j = 0; j < RW; ++j) lc[Cfg::lidx(wpos(q, j))] = v[q * RW + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+#pragma unroll
+      for (int j = 0; j < RR; ++j) v[q * RR + j] = lc[Cfg::lidx(rpos(q, j))];
+    }
+  }
+}
+
 // write the outputs of stage S, read the inputs of stage S+1
-template <typename Real, int N, int RM, int S>
-__device__ __forceinline__ void stage_exchange(cplx_t<Real>* v, int t, cplx_t<Real>* line_lds) {
+template <typename Real, int N, int RM, bool SPLIT, int S>
+__device__ __forceinline__ void stage_exchange(cplx_t<Real>* v, int t, void* line) {
   using Cfg = FastCfg<Real, N, RM>;
   constexpr int r = Cfg::P.rad[S];
   constexpr int L = span_before(N, S);
@@ -415,31 +473,36 @@ __device__ __forceinline__ void stage_exchange(cplx_t<Real>* v, int t, cplx_t<Re
   constexpr int NB = R / r;
   constexpr int r2 = Cfg::P.rad[S + 1];
   constexpr int NB2 = R / r2;
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NB; ++q) {
-    const int beta = t * NB + q;
-    const int k = beta % L;
-    const int base = (beta / L) * L * r + k;
-#pragma unroll
-    for (int j = 0; j < r; ++j) line_lds[Cfg::lidx(base + j * L)] = v[q * r + j];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NB2; ++q) {
-    const int beta = t * NB2 + q;
-#pragma unroll
-    for (int j = 0; j < r2; ++j) v[q * r2 + j] = line_lds[Cfg::lidx(beta + j * (N / r2))];
+  lds_permute<Real, N, RM, SPLIT, NB, r, NB2, r2>(
+      v, line,
+      [t](int q, int j) {
+        const int beta = t * NB + q;
+        return (beta / L) * L * r + beta % L + j * L;
+      },
+      [t](int q, int j) { return t * NB2 + q + j * (N / r2); });
+}
+
+// after the last stage the outputs sit at positions beta + j*(N/rl)
+// (beta = t*(R/rl) + q); regroup them to positions t + j*T
+template <typename Real, int N, int RM, bool SPLIT>
+__device__ __forceinline__ void final_regroup(cplx_t<Real>* v, int t, void* line) {
+  using Cfg = FastCfg<Real, N, RM>;
+  constexpr int R = Cfg::R, T = Cfg::T;
+  constexpr int rl = Cfg::P.rad[Cfg::NST - 1];
+  if constexpr (rl != R) {
+    constexpr int NBl = R / rl;
+    lds_permute<Real, N, RM, SPLIT, NBl, rl, 1, R>(
+        v, line, [t](int q, int j) { return t * NBl + q + j * (N / rl); },
+        [t](int, int j) { return t + j * T; });
   }
 }
 
-template <typename Real, int N, int RM, int S> struct StageLoop {
-  __device__ static __forceinline__ void run(cplx_t<Real>* v, int t, const double2* tw,
-                                             cplx_t<Real>* line_lds) {
+template <typename Real, int N, int RM, bool SPLIT, int S> struct StageLoop {
+  __device__ static __forceinline__ void run(cplx_t<Real>* v, int t, const double2* tw, void* line) {
     stage_compute<Real, N, RM, S>(v, t, tw);
     if constexpr (S + 1 < FastCfg<Real, N, RM>::NST) {
-      stage_exchange<Real, N, RM, S>(v, t, line_lds);
-      StageLoop<Real, N, RM, S + 1>::run(v, t, tw, line_lds);
+      stage_exchange<Real, N, RM, SPLIT, S>(v, t, line);
+      StageLoop<Real, N, RM, SPLIT, S + 1>::run(v, t, tw, line);
     }
   }
 };
@@ -448,11 +511,11 @@ template <typename Real, int N, int RM, int S> struct StageLoop {
 // lanes per line.  Lanes beyond B*T (a block shape shared by several line
 // lengths) run the stages on zeros in a spare LDS slot (index B) so every
 // lane reaches every barrier; they load and store nothing.
-template <typename Real, int N, int RM, bool NT>
-__device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, int B, cplx_t<Real>* lds) {
+template <typename Real, int N, int RM, bool NT, bool SPLIT>
+__device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
   using C = cplx_t<Real>;
   using Cfg = FastCfg<Real, N, RM>;
-  constexpr int R = Cfg::R, T = Cfg::T, NST = Cfg::NST;
+  constexpr int R = Cfg::R, T = Cfg::T;
   const int tid = threadIdx.x;
   int b, t;
   if (p.lines_fast) {
@@ -468,17 +531,40 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
     t = 0;
   }
   const int32_t line = blk * B + b;
-  const bool valid = active && line < p.nlines && !padding_line<Real>(p, line);
   const double2* tw = reinterpret_cast<const double2*>(p.tw);
-  C* line_lds = lds + b * Cfg::LDSN;
+  void* buf = lds + b * line_lds_bytes<Real, N, RM, SPLIT>();
 
+  bool valid;
   LineIO<Real, NT> io;
-  if (valid) io = make_line_io<Real, NT>(p, line);
+  if (T >= 64 && !p.lines_fast) {
+    // lanes along the line with T >= 64: each wave holds one line (or only
+    // spare lanes), so the line index is wave-uniform and the descriptor
+    // arithmetic of make_line_io runs once per wave on the scalar unit
+    const int32_t wl = __builtin_amdgcn_readfirstlane(line);
+    valid = active && wl < p.nlines && !padding_line<Real>(p, wl);
+    if (valid) io = make_line_io<Real, NT>(p, wl);
+  } else {
+    valid = active && line < p.nlines && !padding_line<Real>(p, line);
+    if (valid) io = make_line_io<Real, NT>(p, line);
+  }
 
   C v[R];
-  {
-    constexpr int r0 = Cfg::P.rad[0];
-    constexpr int NB0 = R / r0;
+  constexpr int r0 = Cfg::P.rad[0];
+  constexpr int NB0 = R / r0;
+  if (p.kind == kC2C) {
+    // C2C lines are walked with pointer increments (no 64-bit product per element)
+#pragma unroll
+    for (int q = 0; q < NB0; ++q) {
+      const C* src = reinterpret_cast<const C*>(io.ib) + static_cast<int64_t>(t * NB0 + q) * io.is;
+      const int64_t step = static_cast<int64_t>(N / r0) * io.is;
+#pragma unroll
+      for (int j = 0; j < r0; ++j) {
+        C x = valid ? gload_c<NT, Real>(src) : C{Real(0), Real(0)};
+        v[q * r0 + j] = p.inverse ? cconj(x) : x;
+        src += step;
+      }
+    }
+  } else {
 #pragma unroll
     for (int q = 0; q < NB0; ++q) {
       const int beta = t * NB0 + q;
@@ -486,36 +572,35 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
       for (int j = 0; j < r0; ++j) v[q * r0 + j] = valid ? io.load(beta + j * (N / r0)) : C{Real(0), Real(0)};
     }
   }
-  StageLoop<Real, N, RM, 0>::run(v, t, tw, line_lds);
+  StageLoop<Real, N, RM, SPLIT, 0>::run(v, t, tw, buf);
+  final_regroup<Real, N, RM, SPLIT>(v, t, buf);
 
-  constexpr int rl = Cfg::P.rad[NST - 1];
-  if constexpr (rl != R) {
-    // outputs sit at positions beta + j*(N/rl); regroup to t + j*T through LDS
-    constexpr int NBl = R / rl;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NBl; ++q) {
-      const int beta = t * NBl + q;
-#pragma unroll
-      for (int j = 0; j < rl; ++j) line_lds[Cfg::lidx(beta + j * (N / rl))] = v[q * rl + j];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = line_lds[Cfg::lidx(t + j * T)];
-  }
   if (valid) {
+    if (p.kind == kC2C) {
+      C* dst = reinterpret_cast<C*>(io.ob) + static_cast<int64_t>(t) * io.os;
+      const int64_t step = static_cast<int64_t>(T) * io.os;
 #pragma unroll
-    for (int j = 0; j < R; ++j) io.store(t + j * T, v[j]);
+      for (int j = 0; j < R; ++j) {
+        C x = p.inverse ? cconj(v[j]) : v[j];
+        x.x *= io.scale;
+        x.y *= io.scale;
+        gstore_c<NT, Real>(dst, x);
+        dst += step;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) io.store(t + j * T, v[j]);
+    }
   }
 }
 
 // Paired R2C / C2R: slot b of the block transforms lines 2q and 2q+1
 // (q = blk*B + b) as one complex line; same stage code as fast_line_body.
-template <typename Real, int N, int RM, bool NT>
-__device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, int B, cplx_t<Real>* lds) {
+template <typename Real, int N, int RM, bool NT, bool SPLIT>
+__device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
   using C = cplx_t<Real>;
   using Cfg = FastCfg<Real, N, RM>;
-  constexpr int R = Cfg::R, T = Cfg::T, NST = Cfg::NST;
+  constexpr int R = Cfg::R, T = Cfg::T;
   const int tid = threadIdx.x;
   int b, t;
   if (p.lines_fast) {
@@ -533,10 +618,17 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
   const int32_t pair = blk * B + b;
   const bool valid = active && 2 * pair < p.nlines;  // paired passes never enumerate padding
   const double2* tw = reinterpret_cast<const double2*>(p.tw);
-  C* line_lds = lds + b * Cfg::LDSN;
+  void* buf = lds + b * line_lds_bytes<Real, N, RM, SPLIT>();
 
   PairIO<Real, NT> io;
-  if (valid) io = make_pair_io<Real, NT>(p, pair);
+  if (valid) {
+    // wave-uniform pair index: descriptor arithmetic on the scalar unit (as in fast_line_body)
+    if (T >= 64 && !p.lines_fast) {
+      io = make_pair_io<Real, NT>(p, __builtin_amdgcn_readfirstlane(pair));
+    } else {
+      io = make_pair_io<Real, NT>(p, pair);
+    }
+  }
 
   C v[R];
   {
@@ -549,22 +641,8 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
       for (int j = 0; j < r0; ++j) v[q * r0 + j] = valid ? io.load(beta + j * (N / r0)) : C{Real(0), Real(0)};
     }
   }
-  StageLoop<Real, N, RM, 0>::run(v, t, tw, line_lds);
-
-  constexpr int rl = Cfg::P.rad[NST - 1];
-  if constexpr (rl != R) {
-    constexpr int NBl = R / rl;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NBl; ++q) {
-      const int beta = t * NBl + q;
-#pragma unroll
-      for (int j = 0; j < rl; ++j) line_lds[Cfg::lidx(beta + j * (N / rl))] = v[q * rl + j];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = line_lds[Cfg::lidx(t + j * T)];
-  }
+  StageLoop<Real, N, RM, SPLIT, 0>::run(v, t, tw, buf);
+  final_regroup<Real, N, RM, SPLIT>(v, t, buf);
   if (p.kind == kC2R) {
     if (valid) {
 #pragma unroll
@@ -573,15 +651,43 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
     return;
   }
   // R2C: Z_{n-pos} lives in another lane; exchange the whole line through LDS
-  __syncthreads();
+  C w[R];  // w[j] = Z_{(n - pos) mod n} for pos = t + j*T
+  if constexpr (SPLIT) {
+    Real* lr = static_cast<Real*>(buf);
 #pragma unroll
-  for (int j = 0; j < R; ++j) line_lds[Cfg::lidx(t + j * T)] = v[j];
-  __syncthreads();
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < R; ++j) lr[Cfg::lidx(t + j * T)] = h ? v[j].y : v[j].x;
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int pos = t + j * T;
+        const Real x = lr[Cfg::lidx(pos == 0 ? 0 : N - pos)];
+        if (h) {
+          w[j].y = x;
+        } else {
+          w[j].x = x;
+        }
+      }
+    }
+  } else {
+    C* lc = static_cast<C*>(buf);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) lc[Cfg::lidx(t + j * T)] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int pos = t + j * T;
+      w[j] = lc[Cfg::lidx(pos == 0 ? 0 : N - pos)];
+    }
+  }
   if (valid) {
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int pos = t + j * T;
-      if (2 * pos <= N) io.store_r2c(pos, v[j], line_lds[Cfg::lidx(pos == 0 ? 0 : N - pos)]);
+      if (2 * pos <= N) io.store_r2c(pos, v[j], w[j]);
     }
   }
 }
@@ -608,18 +714,18 @@ __device__ __forceinline__ int32_t pass_block(const LinePass& p, int32_t b, int3
   return b;
 }
 
-template <typename Real, int N, int RM, bool NT>
+template <typename Real, int N, int RM, bool NT, bool SPLIT>
 __global__ void __launch_bounds__(1024) fft_pair_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
   const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
-  fast_pair_body<Real, N, RM, NT>(p, blk, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
+  fast_pair_body<Real, N, RM, NT, SPLIT>(p, blk, p.lines_per_block, smem_raw);
 }
 
-template <typename Real, int N, int RM, bool NT>
+template <typename Real, int N, int RM, bool NT, bool SPLIT>
 __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
   const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
-  fast_line_body<Real, N, RM, NT>(p, blk, p.lines_per_block, reinterpret_cast<cplx_t<Real>*>(smem_raw));
+  fast_line_body<Real, N, RM, NT, SPLIT>(p, blk, p.lines_per_block, smem_raw);
 }
 
 // ---------------------------------------------------------------------------
@@ -634,7 +740,7 @@ __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* 
                                                                 const int32_t* __restrict__ first_block,
                                                                 int32_t count, int32_t total_blocks) {
   extern __shared__ unsigned char smem_raw[];
-  cplx_t<Real>* lds = reinterpret_cast<cplx_t<Real>*>(smem_raw);
+  unsigned char* lds = smem_raw;
   for (int32_t vb = blockIdx.x; vb < total_blocks; vb += gridDim.x) {
     // member of virtual block vb: the last i with first_block[i] <= vb
     int32_t lo = 0, hi = count - 1;
@@ -647,13 +753,13 @@ __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* 
     const LinePass& p = items[i];
     const int32_t blk = vb - first_block[i];
     switch (p.n) {
-      case 16: fast_line_body<Real, 16, 8, NT>(p, blk, p.lines_per_block, lds); break;
-      case 32: fast_line_body<Real, 32, 8, NT>(p, blk, p.lines_per_block, lds); break;
-      case 64: fast_line_body<Real, 64, 8, NT>(p, blk, p.lines_per_block, lds); break;
-      case 96: fast_line_body<Real, 96, 8, NT>(p, blk, p.lines_per_block, lds); break;
-      case 128: fast_line_body<Real, 128, 8, NT>(p, blk, p.lines_per_block, lds); break;
-      case 160: fast_line_body<Real, 160, 8, NT>(p, blk, p.lines_per_block, lds); break;
-      case 256: fast_line_body<Real, 256, 8, NT>(p, blk, p.lines_per_block, lds); break;
+      case 16: fast_line_body<Real, 16, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+      case 32: fast_line_body<Real, 32, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+      case 64: fast_line_body<Real, 64, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+      case 96: fast_line_body<Real, 96, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+      case 128: fast_line_body<Real, 128, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+      case 160: fast_line_body<Real, 160, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+      case 256: fast_line_body<Real, 256, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
       default: break;
     }
     __syncthreads();  // the next virtual block reuses the LDS lines
@@ -784,11 +890,50 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
 // ---------------------------------------------------------------------------
 namespace {
 
+// One compile-time variant of the fast kernels: paired or single lines,
+// nontemporal or cached accesses, split or complex LDS exchanges.
+template <typename Real, int N, int RM, bool NT, bool SPLIT> struct FastVariant {
+  static const void* fn(bool pair) {
+    return pair ? reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, NT, SPLIT>)
+                : reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM, NT, SPLIT>);
+  }
+  static void launch(bool pair, unsigned grid, unsigned threads, size_t smem, hipStream_t s, const LinePass& p) {
+    if (pair) {
+      hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, NT, SPLIT>), dim3(grid), dim3(threads), smem, s, p);
+    } else {
+      hipLaunchKernelGGL((fft_line_fast<Real, N, RM, NT, SPLIT>), dim3(grid), dim3(threads), smem, s, p);
+    }
+  }
+};
+
+template <typename Real, int N, int RM, bool NT>
+void launch_fast_variant(bool split, bool pair, unsigned grid, unsigned threads, size_t smem, hipStream_t s,
+                         const LinePass& p) {
+  // split exchanges exist for FP64 only (FP32 lines are already small in LDS)
+  if constexpr (std::is_same<Real, double>::value) {
+    if (split) {
+      FastVariant<Real, N, RM, NT, true>::launch(pair, grid, threads, smem, s, p);
+      return;
+    }
+  }
+  FastVariant<Real, N, RM, NT, false>::launch(pair, grid, threads, smem, s, p);
+}
+
+template <typename Real, int N, int RM>
+const void* fast_variant_fn(bool split, bool pair, bool nt) {
+  if constexpr (std::is_same<Real, double>::value) {
+    if (split) return nt ? FastVariant<Real, N, RM, true, true>::fn(pair) : FastVariant<Real, N, RM, false, true>::fn(pair);
+  }
+  return nt ? FastVariant<Real, N, RM, true, false>::fn(pair) : FastVariant<Real, N, RM, false, false>::fn(pair);
+}
+
 template <typename Real, int N, int RM>
 hipError_t launch_fast(LinePass p, hipStream_t s) {
   using Cfg = FastCfg<Real, N, RM>;
   using C = cplx_t<Real>;
   constexpr int T = Cfg::T;
+  const bool split = std::is_same<Real, double>::value && p.split_lds != 0;
+  const size_t line_bytes = split ? line_lds_bytes<Real, N, RM, true>() : line_lds_bytes<Real, N, RM, false>();
   int B = p.lines_per_block;
   if (B <= 0) {
     if (p.lines_fast) {
@@ -797,25 +942,23 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
     } else {
       B = T >= 256 ? 1 : 256 / T;
     }
-    while (B > 1 && static_cast<size_t>(B) * Cfg::LDSN * sizeof(C) > 160 * 1024) B /= 2;
+    while (B > 1 && static_cast<size_t>(B) * line_bytes > 160 * 1024) B /= 2;
   }
-  const size_t smem = static_cast<size_t>(B) * Cfg::LDSN * sizeof(C);
+  const size_t smem = static_cast<size_t>(B) * line_bytes;
   if (B * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
   p.lines_per_block = B;
   // two real lines per complex FFT for forward R2C and inverse C2R passes
   // (kernels for 8 values per thread)
-  const int pair = RM == 8 && p.paired && ((p.kind == kR2C && !p.inverse) || (p.kind == kC2R && p.inverse)) ? 1 : 0;
-  const int nt = p.nontemporal ? 1 : 0;
-  const void* fns[4] = {reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM, false>),
-                        reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM, true>),
-                        reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, false>),
-                        reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, true>)};
-  const int v = 2 * pair + nt;
+  const bool pair = RM == 8 && p.paired && ((p.kind == kR2C && !p.inverse) || (p.kind == kC2R && p.inverse));
+  const bool nt = p.nontemporal != 0;
+  const int v = 4 * (split ? 1 : 0) + 2 * (pair ? 1 : 0) + (nt ? 1 : 0);
   // raise the dynamic-LDS limit once per kernel instance (never inside a
   // stream capture that follows an eager launch of the same shape)
-  static size_t lds_limit[4] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
+  static size_t lds_limit[8] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024,
+                                64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
   if (smem > lds_limit[v]) {
-    hipError_t e = hipFuncSetAttribute(fns[v], hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
+    hipError_t e = hipFuncSetAttribute(fast_variant_fn<Real, N, RM>(split, pair, nt),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
     if (e != hipSuccess) return e;
     lds_limit[v] = smem;
   }
@@ -829,11 +972,10 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
     const int32_t C = (inner > 0 && inner % per == 0) ? inner / per : 0;
     p.swizzle = (C > 1 && grid % static_cast<unsigned>(C) == 0 && grid / C > 1) ? C : 0;
   }
-  switch (v) {
-    case 0: hipLaunchKernelGGL((fft_line_fast<Real, N, RM, false>), dim3(grid), dim3(T * B), smem, s, p); break;
-    case 1: hipLaunchKernelGGL((fft_line_fast<Real, N, RM, true>), dim3(grid), dim3(T * B), smem, s, p); break;
-    case 2: hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, false>), dim3(grid), dim3(T * B), smem, s, p); break;
-    default: hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, true>), dim3(grid), dim3(T * B), smem, s, p); break;
+  if (nt) {
+    launch_fast_variant<Real, N, RM, true>(split, pair, grid, static_cast<unsigned>(T * B), smem, s, p);
+  } else {
+    launch_fast_variant<Real, N, RM, false>(split, pair, grid, static_cast<unsigned>(T * B), smem, s, p);
   }
   return hipGetLastError();
 }

@@ -342,6 +342,7 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   p.paired = s.kind == kC2C ? 0 : s.paired;
   p.xcd_remap = s.xcd;
   p.swizzle = s.swz;
+  p.split_lds = s.split;
   p.tw = twiddles(s.n);
   p.lines_per_block = s.lines_per_block;
   p.values_per_thread = s.values_per_thread;
@@ -584,6 +585,7 @@ void Plan::set_option(const std::string& key, long value) {
   else if (field == "pair") passes[i].paired = static_cast<int>(value);
   else if (field == "xcd") passes[i].xcd = static_cast<int>(value);
   else if (field == "swz") passes[i].swz = static_cast<int>(value);
+  else if (field == "split") passes[i].split = static_cast<int>(value);
   else throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -604,6 +606,7 @@ long Plan::get_option(const std::string& key) const {
   if (field == "pair") return passes[i].paired;
   if (field == "xcd") return passes[i].xcd;
   if (field == "swz") return passes[i].swz;
+  if (field == "split") return passes[i].split;
   throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 

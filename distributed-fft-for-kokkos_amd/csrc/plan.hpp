@@ -27,6 +27,7 @@ struct PassSpec {
   int paired = 1;        // R2C / C2R: two real lines per complex FFT
   int xcd = 0;           // XCD-contiguous block numbering
   int swz = 0;           // column-major walk of the block grid
+  int split = 1;         // FP64: exchanges in real / imaginary halves (half the LDS)
 };
 
 // device table exp(-2 pi i e / n), e in [0, n): evaluated in long double,
@@ -49,10 +50,11 @@ class Plan {
   int num_passes(int direction) const;
   // tuning knobs: "chunk" (planes per chunk of the first two passes, 0 = off),
   // "pad" (scratch rows padded to 128-byte multiples: -1 auto, 0 off, 1 on),
-  // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd|swz", "bwd.<i>.…" (lines per block,
-  // values per thread, lanes-across-lines, nontemporal accesses, two real
-  // lines per complex FFT in an R2C / C2R pass, XCD-contiguous blocks,
-  // column-major block walk) of pass i
+  // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd|swz|split", "bwd.<i>.…" (lines per
+  // block, values per thread, lanes-across-lines, nontemporal accesses, two
+  // real lines per complex FFT in an R2C / C2R pass, XCD-contiguous blocks,
+  // column-major block walk, FP64 LDS exchanges in real/imaginary halves) of
+  // pass i
   void set_option(const std::string& key, long value);
   long get_option(const std::string& key) const;
   // measure candidate launch shapes and chunk sizes on the caller's arrays

@@ -147,11 +147,12 @@ int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int d
  * plan scratch padded to 128-byte multiples (-1 auto: when that adds at most
  * 1/16, 0 off, 1 on); "fwd.<i>.lpb",
  * "fwd.<i>.vpt", "fwd.<i>.lf", "fwd.<i>.nt", "fwd.<i>.pair", "fwd.<i>.xcd",
- * "fwd.<i>.swz" (and "bwd.…") = lines per block, values per thread (4, 8,
- * 16), lane order, nontemporal global accesses (-1 = on for arrays of at
- * least 1 GiB), two real lines per complex FFT in an R2C / C2R pass (default
- * on), XCD-contiguous block numbering and a column-major walk of the block
- * grid of pass i. */
+ * "fwd.<i>.swz", "fwd.<i>.split" (and "bwd.…") = lines per block, values
+ * per thread (4, 8, 16), lane order, nontemporal global accesses (-1 = on
+ * for arrays of at least 1 GiB), two real lines per complex FFT in an R2C /
+ * C2R pass (default on), XCD-contiguous block numbering, a column-major walk
+ * of the block grid, and FP64 radix-stage exchanges through LDS in real and
+ * imaginary halves (default on; half the LDS per line) of pass i. */
 int dfft_plan_set_option(dfft_plan* plan, const char* key, long value);
 int dfft_plan_get_option(const dfft_plan* plan, const char* key, long* value);
 /* Autotuner: times every launch shape of every pass and the chunk sizes on the

@@ -194,3 +194,33 @@ def test_padded_scratch_rows(dev, pad, real, fp64):
         plan.set_option("pad", pad)
         plan.set_option("chunk", 3)
     _check(dev, [12, 10, 9], [0, 1, 2], [2, 1, 1], [1, 2, 1], real, fp64, seed=5, configure=chunked)
+
+
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("vpt", [4, 8, 16])
+def test_split_lds_exchanges(dev, real, vpt):
+    """FP64 radix-stage exchanges through LDS in real / imaginary halves (the
+    default) against complex exchanges: both match the oracle, and since only
+    data movement differs they agree to rounding (the compiler may contract
+    multiply-adds differently in the two kernels, so not bit for bit).  Every
+    fast length along contiguous and strided axes, paired R2C / C2R lines
+    included."""
+    for gin, lpb in (([16, 96, 160], 2), ([128, 64, 256], 1), ([512, 8, 32], 4), ([32, 12, 512], 1)):
+        results = []
+        for split in (1, 0):
+            def cfg(plan, ins, outs, split=split):
+                for d in ("fwd", "bwd"):
+                    for i in range(plan.num_passes(0 if d == "fwd" else 1)):
+                        plan.set_option(f"{d}.{i}.split", split)
+                        plan.set_option(f"{d}.{i}.vpt", vpt)
+                        plan.set_option(f"{d}.{i}.lpb", lpb)
+            x = seeded_input(gin, real, 11)
+            outs, backs, ins, _ = run_gpu_plan(dev, x, gin, [0, 1, 2], [2, 1, 1], [1, 2, 1], real, True,
+                                               configure=cfg)
+            ref = reference_spectrum(x, gin, [0, 1, 2], [1, 2, 1], real)
+            assert gather_rel_l2(outs, ref) <= TOL64, f"split={split} {gin} vpt={vpt}"
+            assert gather_rel_l2(backs, ins) <= TOL64, f"split={split} {gin} vpt={vpt}"
+            results.append((outs, backs))
+        for k in range(2):
+            e = gather_rel_l2(results[0][k], results[1][k])
+            assert e <= 1e-14, f"split and complex exchanges differ by {e:.2e} for {gin} vpt={vpt}"
