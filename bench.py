@@ -13,6 +13,8 @@ on the launch stream; `cpu_baseline` times the CPU restatement of the plan
 (oracle, single thread for the FFT lines) on a bounded sample and scales it.
 
 Further sections of the same line:
+* r2c_1024: 1024^3 R2C forward + C2R backward, slab (4,1,1) -> (1,4,1),
+  FP64 and FP32, autotuned, with per-pass algorithmic bytes and HBM fraction;
 * ns: Taylor-Green vortex 256^3 float64, Re = 1600, dt = 1e-3, 2x2 virtual
   ranks, divergence form (3 velocity + 6 product fields per RK stage, each set
   in one batched transform), steps replayed from a captured hipGraph;
@@ -240,6 +242,57 @@ def bench_mixed_batch(torch, dfft, dev, steps, warmup):
             "roundtrip_rel_l2_max": err}
 
 
+def bench_r2c_1024(torch, dfft, dev, steps, warmup, tune):
+    """1024^3 R2C forward + C2R backward, slab (4,1,1) -> (1,4,1), FP64 and FP32,
+    U[-1,1] real inputs; per-pass algorithmic bytes: R2C / C2R passes move the
+    real array and the half spectrum, C2C passes read and write the half
+    spectrum."""
+    n, P_ = 1024, 4
+    gin, gout = [n, n, n], [n, n, n // 2 + 1]
+    res = {"workload": "3D R2C/C2R FFT 1024^3, slab (4,1,1)->(1,4,1), 4 virtual ranks, fwd+bwd"}
+    for tag, rdt, cdt in (("fp64", torch.float64, torch.complex128), ("fp32", torch.float32, torch.complex64)):
+        log(f"[bench] r2c 1024^3 {tag} ...")
+        gen = torch.Generator(device=dev).manual_seed(7)
+
+        def inputs():
+            return [torch.rand(dfft.compute_local_extents_and_starts(gin, [P_, 1, 1], r)[0], generator=gen,
+                               device=dev, dtype=rdt) * 2 - 1 for r in range(P_)]
+        ins = inputs()
+        outs = [torch.empty(dfft.compute_local_extents_and_starts(gout, [1, P_, 1], r)[0], dtype=cdt, device=dev)
+                for r in range(P_)]
+        plan = dfft.Plan(ins, outs, [0, 1, 2], [P_, 1, 1], [1, P_, 1], global_in_extents=gin,
+                         global_out_extents=gout)
+        if tune:
+            t0 = time.perf_counter()
+            plan.autotune(ins, outs)
+            torch.cuda.synchronize()
+            log(f"[bench] r2c {tag}: autotune {time.perf_counter() - t0:.1f} s")
+            del ins
+            ins = inputs()
+        backs = [torch.empty_like(a) for a in ins]
+        ms = time_roundtrip(torch, dfft, plan, ins, outs, backs, steps, warmup)
+        fw, bw = pass_times(dfft, plan, ins, outs, backs, max(2, steps // 2))
+        err = math.sqrt(sum(float(torch.linalg.vector_norm(b - a)) ** 2 for a, b in zip(ins, backs)) /
+                        sum(float(torch.linalg.vector_norm(a)) ** 2 for a in ins))
+        rs = 8 if tag == "fp64" else 4
+        real_b = n ** 3 * rs
+        half_b = n * n * (n // 2 + 1) * 2 * rs
+        # forward: R2C, C2C, C2C; backward: C2C, C2C, C2R
+        pass_bytes = [real_b + half_b, 2 * half_b, 2 * half_b, 2 * half_b, 2 * half_b, half_b + real_b]
+        allp = fw + bw
+        achieved = sum(pass_bytes) / (sum(allp) * 1e-3) / 1e9
+        res[tag] = {"ms": round(ms, 3), "fwd_pass_ms": [round(x, 4) for x in fw],
+                    "bwd_pass_ms": [round(x, 4) for x in bw],
+                    "avg_pass_GBs": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "roofline_2B": round(2 * (real_b + half_b) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "roundtrip_rel_l2": err}
+        log(f"[bench] r2c {tag}: {ms:.3f} ms, passes {fw} {bw}, {achieved:.0f} GB/s per pass")
+        del plan, ins, outs, backs
+        torch.cuda.empty_cache()
+    res["fp32_speedup"] = round(res["fp64"]["ms"] / res["fp32"]["ms"], 3)
+    return res
+
+
 def bench_precision(torch, dfft, dev):
     """FP32 vs FP64 (both on the GPU; the FP64 path agrees with the CPU oracle to 1e-12
     wherever the oracle runs) and vs the exact spectrum of a single mode."""
@@ -313,8 +366,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the plan autotuner")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
                     help="run one precision only (profiling runs)")
-    ap.add_argument("--sections", default="fft,ns,adjoint,batch,precision,rocfft",
-                    help="comma list of: fft (headline), ns, adjoint, batch, precision, rocfft")
+    ap.add_argument("--sections", default="fft,r2c,ns,adjoint,batch,precision,rocfft",
+                    help="comma list of: fft (headline), r2c, ns, adjoint, batch, precision, rocfft")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -428,6 +481,10 @@ def run_sections(torch, dfft, dev, args, sections, line):
     if "batch" in sections:
         log("[bench] mixed batch of 64 transforms ...")
         line["mixed_batch"] = bench_mixed_batch(torch, dfft, dev, args.steps, args.warmup)
+        torch.cuda.empty_cache()
+    if "r2c" in sections:
+        line["r2c_1024"] = bench_r2c_1024(torch, dfft, dev, max(3, args.steps // 2), max(1, args.warmup),
+                                          not args.no_tune)
         torch.cuda.empty_cache()
     if "precision" in sections:
         log("[bench] precision FP32 vs FP64 ...")

@@ -229,19 +229,10 @@ template <typename Real, bool NT = false> struct PairIO {
   Real hw;     // interior half-spectrum weight (1: none)
   bool has_b;  // the pair's second line exists
 
-  // value fed to the forward FFT at position pos (C2R lines are conjugated:
-  // inverse = conj o FFT o conj)
-  __device__ __forceinline__ C load(int32_t pos) const {
-    if (kind == kR2C) {
-      const Real a = gload<NT>(reinterpret_cast<const Real*>(ib[0]) + pos * is[0]);
-      const Real b = has_b ? gload<NT>(reinterpret_cast<const Real*>(ib[1]) + pos * is[1]) : Real(0);
-      return C{a, b};
-    }
-    const int32_t nh = n / 2 + 1;
-    const bool lo = pos < nh;
-    const int32_t src = lo ? pos : n - pos;
-    C A = gload_c<NT, Real>(reinterpret_cast<const C*>(ib[0]) + src * is[0]);
-    C Bv = has_b ? gload_c<NT, Real>(reinterpret_cast<const C*>(ib[1]) + src * is[1]) : C{Real(0), Real(0)};
+  // C2R input at position pos (C2R lines are conjugated: inverse = conj o
+  // FFT o conj) from the loaded half-spectrum entries A = X_a[src], Bv =
+  // X_b[src], src = pos below n/2+1 (lo) and n - pos above
+  __device__ __forceinline__ C c2r_value(int32_t pos, bool lo, C A, C Bv) const {
     if (!lo) {
       A = cconj(A);
       Bv = cconj(Bv);
@@ -258,20 +249,11 @@ template <typename Real, bool NT = false> struct PairIO {
     // conj(A + i B)
     return C{A.x - Bv.y, -(A.y + Bv.x)};
   }
-  // C2R: w = FFT(conj Z) at position pos, so a + i b = conj(w)
-  __device__ __forceinline__ void store_c2r(int32_t pos, C w) const {
-    gstore<NT>(reinterpret_cast<Real*>(ob[0]) + pos * os[0], w.x * scale);
-    if (has_b) gstore<NT>(reinterpret_cast<Real*>(ob[1]) + pos * os[1], -w.y * scale);
-  }
   // R2C: spectra of both lines at pos <= n/2 from Z_pos and Z_{n-pos}
-  __device__ __forceinline__ void store_r2c(int32_t pos, C zk, C zm) const {
+  __device__ __forceinline__ void r2c_values(int32_t pos, C zk, C zm, C& A, C& Bv) const {
     const Real h = Real(0.5) * scale * ((pos != 0 && 2 * pos != n) ? hw : Real(1));
-    const C A{(zk.x + zm.x) * h, (zk.y - zm.y) * h};
-    gstore_c<NT, Real>(reinterpret_cast<C*>(ob[0]) + pos * os[0], A);
-    if (has_b) {
-      const C Bv{(zk.y + zm.y) * h, (zm.x - zk.x) * h};
-      gstore_c<NT, Real>(reinterpret_cast<C*>(ob[1]) + pos * os[1], Bv);
-    }
+    A = C{(zk.x + zm.x) * h, (zk.y - zm.y) * h};
+    Bv = C{(zk.y + zm.y) * h, (zm.x - zk.x) * h};
   }
 };
 
@@ -620,7 +602,7 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
   const double2* tw = reinterpret_cast<const double2*>(p.tw);
   void* buf = lds + b * line_lds_bytes<Real, N, RM, SPLIT>();
 
-  PairIO<Real, NT> io;
+  PairIO<Real, NT> io{};
   if (valid) {
     // wave-uniform pair index: descriptor arithmetic on the scalar unit (as in fast_line_body)
     if (T >= 64 && !p.lines_fast) {
@@ -630,23 +612,66 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
     }
   }
 
+  // line elements are walked with pointer increments (positions beta + j*S
+  // on load, t + j*T on store): no 64-bit product per element
   C v[R];
   {
     constexpr int r0 = Cfg::P.rad[0];
     constexpr int NB0 = R / r0;
+    constexpr int S = N / r0;
+    const bool vb = valid && io.has_b;
+    const int64_t sa = static_cast<int64_t>(S) * io.is[0], sb = static_cast<int64_t>(S) * io.is[1];
 #pragma unroll
     for (int q = 0; q < NB0; ++q) {
       const int beta = t * NB0 + q;
+      if (p.kind == kR2C) {
+        const Real* pa = reinterpret_cast<const Real*>(io.ib[0]) + static_cast<int64_t>(beta) * io.is[0];
+        const Real* pb = reinterpret_cast<const Real*>(io.ib[1]) + static_cast<int64_t>(beta) * io.is[1];
 #pragma unroll
-      for (int j = 0; j < r0; ++j) v[q * r0 + j] = valid ? io.load(beta + j * (N / r0)) : C{Real(0), Real(0)};
+        for (int j = 0; j < r0; ++j) {
+          const Real xa = valid ? gload<NT>(pa) : Real(0);
+          const Real xb = vb ? gload<NT>(pb) : Real(0);
+          v[q * r0 + j] = C{xa, xb};
+          pa += sa;
+          pb += sb;
+        }
+      } else {
+        // C2R: X[pos] below n/2+1 (walking up from beta), conj X[n-pos] above
+        // (walking down from n-beta)
+        const C* la = reinterpret_cast<const C*>(io.ib[0]) + static_cast<int64_t>(beta) * io.is[0];
+        const C* lb = reinterpret_cast<const C*>(io.ib[1]) + static_cast<int64_t>(beta) * io.is[1];
+        const C* ha = reinterpret_cast<const C*>(io.ib[0]) + static_cast<int64_t>(N - beta) * io.is[0];
+        const C* hb = reinterpret_cast<const C*>(io.ib[1]) + static_cast<int64_t>(N - beta) * io.is[1];
+#pragma unroll
+        for (int j = 0; j < r0; ++j) {
+          const int pos = beta + j * S;
+          const bool lo = pos < N / 2 + 1;
+          const C A = valid ? gload_c<NT, Real>(lo ? la : ha) : C{Real(0), Real(0)};
+          const C Bv = vb ? gload_c<NT, Real>(lo ? lb : hb) : C{Real(0), Real(0)};
+          v[q * r0 + j] = io.c2r_value(pos, lo, A, Bv);
+          la += sa;
+          lb += sb;
+          ha -= sa;
+          hb -= sb;
+        }
+      }
     }
   }
   StageLoop<Real, N, RM, SPLIT, 0>::run(v, t, tw, buf);
   final_regroup<Real, N, RM, SPLIT>(v, t, buf);
+  const int64_t ta = static_cast<int64_t>(T) * io.os[0], tb = static_cast<int64_t>(T) * io.os[1];
   if (p.kind == kC2R) {
+    // w = FFT(conj Z) at each position, so a + i b = conj(w)
     if (valid) {
+      Real* pa = reinterpret_cast<Real*>(io.ob[0]) + static_cast<int64_t>(t) * io.os[0];
+      Real* pb = reinterpret_cast<Real*>(io.ob[1]) + static_cast<int64_t>(t) * io.os[1];
 #pragma unroll
-      for (int j = 0; j < R; ++j) io.store_c2r(t + j * T, v[j]);
+      for (int j = 0; j < R; ++j) {
+        gstore<NT>(pa, v[j].x * io.scale);
+        if (io.has_b) gstore<NT>(pb, -v[j].y * io.scale);
+        pa += ta;
+        pb += tb;
+      }
     }
     return;
   }
@@ -684,10 +709,19 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
     }
   }
   if (valid) {
+    C* pa = reinterpret_cast<C*>(io.ob[0]) + static_cast<int64_t>(t) * io.os[0];
+    C* pb = reinterpret_cast<C*>(io.ob[1]) + static_cast<int64_t>(t) * io.os[1];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int pos = t + j * T;
-      if (2 * pos <= N) io.store_r2c(pos, v[j], w[j]);
+      if (2 * pos <= N) {
+        C A, Bv;
+        io.r2c_values(pos, v[j], w[j], A, Bv);
+        gstore_c<NT, Real>(pa, A);
+        if (io.has_b) gstore_c<NT, Real>(pb, Bv);
+      }
+      pa += ta;
+      pb += tb;
     }
   }
 }

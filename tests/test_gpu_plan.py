@@ -224,3 +224,21 @@ def test_split_lds_exchanges(dev, real, vpt):
         for k in range(2):
             e = gather_rel_l2(results[0][k], results[1][k])
             assert e <= 1e-14, f"split and complex exchanges differ by {e:.2e} for {gin} vpt={vpt}"
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+@pytest.mark.parametrize("pad", [0, 1])
+def test_lanes_across_lines_blocks(dev, fp64, pad):
+    """Lanes-across-lines blocks of 1..16 lines on every pass (blocks that end
+    inside a row of the fastest batch axis, cross rank blocks, or hold padding
+    lines of padded scratch rows) against the oracle, in slab and pencil plans."""
+    for lpb in (1, 2, 4, 8, 16):
+        def cfg(plan, ins, outs, lpb=lpb):
+            plan.set_option("pad", pad)
+            for d in ("fwd", "bwd"):
+                for i in range(plan.num_passes(0 if d == "fwd" else 1)):
+                    plan.set_option(f"{d}.{i}.lf", 1)
+                    plan.set_option(f"{d}.{i}.lpb", lpb)
+        _check(dev, [16, 24, 40], [0, 1, 2], [2, 1, 1], [1, 2, 1], False, fp64, seed=21, configure=cfg)
+        _check(dev, [32, 12, 20], [0, 1, 2], [1, 1, 2], [2, 1, 1], True, fp64, seed=22, configure=cfg)
+        _check(dev, [16, 32, 9], [0, 1, 2], [2, 2, 1], [1, 2, 2], False, fp64, seed=23, configure=cfg)
