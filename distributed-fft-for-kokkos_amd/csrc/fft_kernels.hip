@@ -603,14 +603,7 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
   void* buf = lds + b * line_lds_bytes<Real, N, RM, SPLIT>();
 
   PairIO<Real, NT> io{};
-  if (valid) {
-    // wave-uniform pair index: descriptor arithmetic on the scalar unit (as in fast_line_body)
-    if (T >= 64 && !p.lines_fast) {
-      io = make_pair_io<Real, NT>(p, __builtin_amdgcn_readfirstlane(pair));
-    } else {
-      io = make_pair_io<Real, NT>(p, pair);
-    }
-  }
+  if (valid) io = make_pair_io<Real, NT>(p, pair);
 
   // line elements are walked with pointer increments (positions beta + j*S
   // on load, t + j*T on store): no 64-bit product per element
