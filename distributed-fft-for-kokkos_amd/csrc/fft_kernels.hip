@@ -608,6 +608,7 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
   // line elements are walked with pointer increments (positions beta + j*S
   // on load, t + j*T on store): no 64-bit product per element
   C v[R];
+  C mirror[R];  // C2R: input at position n - pos, formed by the lane that loads X[pos]
   {
     constexpr int r0 = Cfg::P.rad[0];
     constexpr int NB0 = R / r0;
@@ -629,23 +630,76 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
           pb += sb;
         }
       } else {
-        // C2R: X[pos] below n/2+1 (walking up from beta), conj X[n-pos] above
-        // (walking down from n-beta)
+        // C2R: each half-spectrum entry X[m], m <= n/2, is loaded once, by the
+        // lane holding position m; that lane also forms the input at the
+        // mirrored position n - m (from conj X[m]) and hands it over through
+        // LDS below
         const C* la = reinterpret_cast<const C*>(io.ib[0]) + static_cast<int64_t>(beta) * io.is[0];
         const C* lb = reinterpret_cast<const C*>(io.ib[1]) + static_cast<int64_t>(beta) * io.is[1];
-        const C* ha = reinterpret_cast<const C*>(io.ib[0]) + static_cast<int64_t>(N - beta) * io.is[0];
-        const C* hb = reinterpret_cast<const C*>(io.ib[1]) + static_cast<int64_t>(N - beta) * io.is[1];
 #pragma unroll
         for (int j = 0; j < r0; ++j) {
           const int pos = beta + j * S;
-          const bool lo = pos < N / 2 + 1;
-          const C A = valid ? gload_c<NT, Real>(lo ? la : ha) : C{Real(0), Real(0)};
-          const C Bv = vb ? gload_c<NT, Real>(lo ? lb : hb) : C{Real(0), Real(0)};
-          v[q * r0 + j] = io.c2r_value(pos, lo, A, Bv);
+          if (2 * pos <= N) {
+            const C A = valid ? gload_c<NT, Real>(la) : C{Real(0), Real(0)};
+            const C Bv = vb ? gload_c<NT, Real>(lb) : C{Real(0), Real(0)};
+            v[q * r0 + j] = io.c2r_value(pos, true, A, Bv);
+            mirror[q * r0 + j] = io.c2r_value(N - pos, false, A, Bv);
+          }
           la += sa;
           lb += sb;
-          ha -= sa;
-          hb -= sb;
+        }
+      }
+    }
+    if (p.kind == kC2R) {
+      // positions above n/2 receive the value formed by the lane holding n - pos
+      if constexpr (SPLIT) {
+        Real* lr = static_cast<Real*>(buf);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (h) __syncthreads();
+#pragma unroll
+          for (int q = 0; q < NB0; ++q) {
+#pragma unroll
+            for (int j = 0; j < r0; ++j) {
+              const int pos = t * NB0 + q + j * S;
+              if (pos > 0 && 2 * pos < N) lr[Cfg::lidx(N - pos)] = h ? mirror[q * r0 + j].y : mirror[q * r0 + j].x;
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < NB0; ++q) {
+#pragma unroll
+            for (int j = 0; j < r0; ++j) {
+              const int pos = t * NB0 + q + j * S;
+              if (2 * pos > N) {
+                const Real x = lr[Cfg::lidx(pos)];
+                if (h) {
+                  v[q * r0 + j].y = x;
+                } else {
+                  v[q * r0 + j].x = x;
+                }
+              }
+            }
+          }
+        }
+      } else {
+        C* lc = static_cast<C*>(buf);
+#pragma unroll
+        for (int q = 0; q < NB0; ++q) {
+#pragma unroll
+          for (int j = 0; j < r0; ++j) {
+            const int pos = t * NB0 + q + j * S;
+            if (pos > 0 && 2 * pos < N) lc[Cfg::lidx(N - pos)] = mirror[q * r0 + j];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NB0; ++q) {
+#pragma unroll
+          for (int j = 0; j < r0; ++j) {
+            const int pos = t * NB0 + q + j * S;
+            if (2 * pos > N) v[q * r0 + j] = lc[Cfg::lidx(pos)];
+          }
         }
       }
     }
@@ -920,24 +974,34 @@ namespace {
 // One compile-time variant of the fast kernels: paired or single lines,
 // nontemporal or cached accesses, split or complex LDS exchanges.
 template <typename Real, int N, int RM, bool NT, bool SPLIT> struct FastVariant {
+  static constexpr bool kPairs = !(std::is_same<Real, double>::value && (N % 3 == 0 || N % 5 == 0));
   static const void* fn(bool pair) {
-    return pair ? reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, NT, SPLIT>)
-                : reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM, NT, SPLIT>);
+    if constexpr (kPairs) {
+      if (pair) return reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, NT, SPLIT>);
+    }
+    return reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM, NT, SPLIT>);
   }
   static void launch(bool pair, unsigned grid, unsigned threads, size_t smem, hipStream_t s, const LinePass& p) {
-    if (pair) {
-      hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, NT, SPLIT>), dim3(grid), dim3(threads), smem, s, p);
-    } else {
-      hipLaunchKernelGGL((fft_line_fast<Real, N, RM, NT, SPLIT>), dim3(grid), dim3(threads), smem, s, p);
+    if constexpr (kPairs) {
+      if (pair) {
+        hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, NT, SPLIT>), dim3(grid), dim3(threads), smem, s, p);
+        return;
+      }
     }
+    hipLaunchKernelGGL((fft_line_fast<Real, N, RM, NT, SPLIT>), dim3(grid), dim3(threads), smem, s, p);
   }
 };
+
+// split exchanges exist for FP64, and for FP32 lines of 1024 points and more
+// (shorter FP32 lines are small in LDS already and ran slower split)
+template <typename Real, int N> constexpr bool split_available() {
+  return std::is_same<Real, double>::value || N >= 1024;
+}
 
 template <typename Real, int N, int RM, bool NT>
 void launch_fast_variant(bool split, bool pair, unsigned grid, unsigned threads, size_t smem, hipStream_t s,
                          const LinePass& p) {
-  // split exchanges exist for FP64 only (FP32 lines are already small in LDS)
-  if constexpr (std::is_same<Real, double>::value) {
+  if constexpr (split_available<Real, N>()) {
     if (split) {
       FastVariant<Real, N, RM, NT, true>::launch(pair, grid, threads, smem, s, p);
       return;
@@ -948,7 +1012,7 @@ void launch_fast_variant(bool split, bool pair, unsigned grid, unsigned threads,
 
 template <typename Real, int N, int RM>
 const void* fast_variant_fn(bool split, bool pair, bool nt) {
-  if constexpr (std::is_same<Real, double>::value) {
+  if constexpr (split_available<Real, N>()) {
     if (split) return nt ? FastVariant<Real, N, RM, true, true>::fn(pair) : FastVariant<Real, N, RM, false, true>::fn(pair);
   }
   return nt ? FastVariant<Real, N, RM, true, false>::fn(pair) : FastVariant<Real, N, RM, false, false>::fn(pair);
@@ -959,7 +1023,7 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
   using Cfg = FastCfg<Real, N, RM>;
   using C = cplx_t<Real>;
   constexpr int T = Cfg::T;
-  const bool split = std::is_same<Real, double>::value && p.split_lds != 0;
+  const bool split = split_available<Real, N>() && p.split_lds != 0;
   const size_t line_bytes = split ? line_lds_bytes<Real, N, RM, true>() : line_lds_bytes<Real, N, RM, false>();
   int B = p.lines_per_block;
   if (B <= 0) {
@@ -976,7 +1040,10 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
   p.lines_per_block = B;
   // two real lines per complex FFT for forward R2C and inverse C2R passes
   // (kernels for 8 values per thread)
-  const bool pair = RM == 8 && p.paired && ((p.kind == kR2C && !p.inverse) || (p.kind == kC2R && p.inverse));
+  // (not for FP64 lengths with an odd factor: the pair kernel's two sets of
+  // 20-40 complex values per lane spill registers there)
+  const bool pair = FastVariant<Real, N, RM, false, false>::kPairs && RM == 8 && p.paired &&
+                    ((p.kind == kR2C && !p.inverse) || (p.kind == kC2R && p.inverse));
   const bool nt = p.nontemporal != 0;
   const int v = 4 * (split ? 1 : 0) + 2 * (pair ? 1 : 0) + (nt ? 1 : 0);
   // raise the dynamic-LDS limit once per kernel instance (never inside a

@@ -49,9 +49,10 @@ struct LinePass {
   // 1: global loads and stores of the line data are nontemporal (streaming
   // arrays larger than the caches; measured 5-13 % faster per pass at 512^3)
   int32_t nontemporal;
-  // 1 (FP64 fast kernels only): the radix stages exchange real parts, then
-  // imaginary parts, through a line buffer of reals, halving the LDS of a
-  // line (more resident blocks per CU for a few more barriers)
+  // 1 (FP64 fast kernels, FP32 ones of length >= 1024): the radix stages
+  // exchange real parts, then imaginary parts, through a line buffer of
+  // reals, halving the LDS of a line (more resident blocks per CU for a few
+  // more barriers)
   int32_t split_lds;
   // Weight of the interior half-spectrum entries (0 < k < n/2) of an R2C /
   // C2R line, applied to the values a C2R line loads or an R2C line stores;

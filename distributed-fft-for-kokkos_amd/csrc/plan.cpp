@@ -716,11 +716,20 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
       float best = 1e30f;
       PassSpec best_spec = passes[i];
       for (std::size_t f = 0; f < finalists; ++f) {
-        float ms = timed[f].first;
-        for (int trial = 0; trial < 2; ++trial) ms = std::min(ms, time_spec(timed[f].second));
-        if (ms < best) {
-          best = ms;
-          best_spec = timed[f].second;
+        // each finalist also with the other LDS exchange form (split halves
+        // or complex; kernels without a split form ignore the option)
+        for (int flip = 0; flip < 2; ++flip) {
+          PassSpec s = timed[f].second;
+          if (flip) s.split = s.split ? 0 : 1;
+          float ms = flip ? 1e30f : timed[f].first;
+          for (int trial = 0; trial < (flip ? 3 : 2); ++trial) {
+            const float m = time_spec(s);
+            if (m >= 0.f) ms = std::min(ms, m);
+          }
+          if (ms < best) {
+            best = ms;
+            best_spec = s;
+          }
         }
       }
       passes[i] = best_spec;

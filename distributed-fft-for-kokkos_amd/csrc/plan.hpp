@@ -27,7 +27,7 @@ struct PassSpec {
   int paired = 1;        // R2C / C2R: two real lines per complex FFT
   int xcd = 0;           // XCD-contiguous block numbering
   int swz = 0;           // column-major walk of the block grid
-  int split = 1;         // FP64: exchanges in real / imaginary halves (half the LDS)
+  int split = 1;         // exchanges in real / imaginary halves (half the LDS; FP64, FP32 n >= 1024)
 };
 
 // device table exp(-2 pi i e / n), e in [0, n): evaluated in long double,
