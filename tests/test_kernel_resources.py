@@ -1,0 +1,97 @@
+"""Resource use of the built line-pass kernels (CPU test on libdfft.so).
+
+Reads the gfx950 code object that hipcc embedded in libdfft.so (the
+.hip_fatbin offload bundle) and its AMDGPU metadata note, and checks the
+private (scratch) memory of every kernel.  The FFT kernels keep their lines
+in registers and LDS; a kernel that needs more than a few spill slots of
+private memory per lane is a code-generation regression (for instance the
+compiler copying the 1.6 KB kernel argument into private memory when it
+cannot address it in place, which made a 1024-point pass ten times slower).
+"""
+
+import struct
+from pathlib import Path
+
+import pytest
+
+msgpack = pytest.importorskip("msgpack")
+
+LIB = Path(__file__).resolve().parents[1] / "distributed-fft-for-kokkos_amd" / "libdfft.so"
+NT_AMDGPU_METADATA = 32
+
+
+def _sections(elf):
+    shoff = struct.unpack_from("<Q", elf, 0x28)[0]
+    shentsize, shnum, shstrndx = struct.unpack_from("<HHH", elf, 0x3A)
+    raw = []
+    for i in range(shnum):
+        name, typ, _flags, _addr, offset, size = struct.unpack_from("<IIQQQQ", elf, shoff + i * shentsize)
+        raw.append((name, typ, offset, size))
+    stroff = raw[shstrndx][2]
+
+    def nm(n):
+        return elf[stroff + n: elf.index(b"\0", stroff + n)].decode()
+    return [(nm(n), typ, off, size) for n, typ, off, size in raw]
+
+
+def _code_object(lib_bytes, arch="gfx950"):
+    fat = [s for s in _sections(lib_bytes) if s[0] == ".hip_fatbin"]
+    assert fat, "libdfft.so has no .hip_fatbin section"
+    _, _, off, size = fat[0]
+    blob = lib_bytes[off: off + size]
+    magic = b"__CLANG_OFFLOAD_BUNDLE__"
+    assert blob.startswith(magic), "unexpected offload bundle format"
+    (count,) = struct.unpack_from("<Q", blob, len(magic))
+    pos = len(magic) + 8
+    for _ in range(count):
+        eoff, esize, tlen = struct.unpack_from("<QQQ", blob, pos)
+        triple = blob[pos + 24: pos + 24 + tlen].decode()
+        pos += 24 + tlen
+        if triple.endswith(arch):
+            return blob[eoff: eoff + esize]
+    raise AssertionError(f"no {arch} code object in the bundle")
+
+
+def _kernels(co):
+    for name, typ, off, size in _sections(co):
+        if typ != 7:  # SHT_NOTE
+            continue
+        p = off
+        while p < off + size:
+            namesz, descsz, ntype = struct.unpack_from("<III", co, p)
+            nname = co[p + 12: p + 12 + namesz].rstrip(b"\0")
+            dpos = p + 12 + ((namesz + 3) & ~3)
+            if nname == b"AMDGPU" and ntype == NT_AMDGPU_METADATA:
+                meta = msgpack.unpackb(co[dpos: dpos + descsz], raw=False)
+                return meta["amdhsa.kernels"]
+            p = dpos + ((descsz + 3) & ~3)
+    raise AssertionError("no AMDGPU metadata note in the code object")
+
+
+@pytest.fixture(scope="module")
+def kernels():
+    if not LIB.exists():
+        pytest.skip("libdfft.so is not built")
+    return _kernels(_code_object(LIB.read_bytes()))
+
+
+def test_metadata_lists_the_line_kernels(kernels):
+    names = [k[".name"] for k in kernels]
+    assert any("fft_line_fast" in n for n in names)
+    assert any("fft_pair_fast" in n for n in names)
+    assert any("fft_group_pass" in n for n in names)
+
+
+def test_no_private_memory_in_power_of_two_line_kernels(kernels):
+    """Fast kernels of lengths 256..2048 (the large-array passes) use no more
+    than a few spill slots; any kernel stays far below a copy of its argument."""
+    big = []
+    for k in kernels:
+        name, priv = k[".name"], k[".private_segment_fixed_size"]
+        if ("fft_line_fast" in name or "fft_pair_fast" in name) and any(
+                f", {n}," in name or f"Li{n}E" in name for n in (256, 512, 1024, 2048)):
+            if priv > 64:
+                big.append((name, priv))
+        elif priv > 512:
+            big.append((name, priv))
+    assert not big, f"kernels with private memory: {big}"
