@@ -13,6 +13,7 @@
 
 #include "fft_kernels.hpp"
 
+#include <cstdint>
 #include <type_traits>
 
 namespace dfft {
@@ -489,6 +490,162 @@ template <typename Real, int N, int RM, bool SPLIT, int S> struct StageLoop {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Two lines per lane (FP32 lanes-across-lines passes whose consecutive lines
+// are adjacent elements of one row): lane b of a block holds lines 2b and
+// 2b+1, one 16-byte access moves element k of both, and the two register
+// sets share twiddles and barriers.  B = 32 lines per block reads 256-byte
+// row segments with 16 lanes across the lines; the LDS exchanges always move
+// real and imaginary halves (2 lines x N + N/16 + 1 floats per lane pair).
+// ---------------------------------------------------------------------------
+template <typename Real, int N, int RM, int S>
+__device__ __forceinline__ void stage_compute2(cplx_t<Real>* va, cplx_t<Real>* vb, int t, const double2* tw) {
+  using Cfg = FastCfg<Real, N, RM>;
+  using C = cplx_t<Real>;
+  constexpr int r = Cfg::P.rad[S];
+  constexpr int L = span_before(N, S);
+  constexpr int R = Cfg::R;
+  constexpr int NB = R / r;
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    if constexpr (L > 1) {
+      const int beta = t * NB + q;
+      const int k = beta % L;
+      constexpr int ts = N / (L * r);
+      double2 pw[8];
+      twiddle_powers<r>(tw[k * ts], pw);
+#pragma unroll
+      for (int j = 1; j < r; ++j) {
+        const C w = to_prec<C>(pw[j]);
+        va[q * r + j] = cmul(va[q * r + j], w);
+        vb[q * r + j] = cmul(vb[q * r + j], w);
+      }
+    }
+    bfly<r>(va + q * r);
+    bfly<r>(vb + q * r);
+  }
+}
+
+// lds_permute for the two register sets of a lane (buffers la, lb of reals)
+template <typename Real, int N, int RM, int NW, int RW, int NR, int RR, typename WPos, typename RPos>
+__device__ __forceinline__ void lds_permute2(cplx_t<Real>* va, cplx_t<Real>* vb, Real* la, Real* lb, WPos wpos,
+                                             RPos rpos) {
+  using Cfg = FastCfg<Real, N, RM>;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+#pragma unroll
+      for (int j = 0; j < RW; ++j) {
+        const int i = Cfg::lidx(wpos(q, j));
+        la[i] = h ? va[q * RW + j].y : va[q * RW + j].x;
+        lb[i] = h ? vb[q * RW + j].y : vb[q * RW + j].x;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+#pragma unroll
+      for (int j = 0; j < RR; ++j) {
+        const int i = Cfg::lidx(rpos(q, j));
+        const Real xa = la[i], xb = lb[i];
+        if (h) {
+          va[q * RR + j].y = xa;
+          vb[q * RR + j].y = xb;
+        } else {
+          va[q * RR + j].x = xa;
+          vb[q * RR + j].x = xb;
+        }
+      }
+    }
+  }
+}
+
+template <typename Real, int N, int RM, int S> struct StageLoop2 {
+  __device__ static __forceinline__ void run(cplx_t<Real>* va, cplx_t<Real>* vb, int t, const double2* tw, Real* la,
+                                             Real* lb) {
+    using Cfg = FastCfg<Real, N, RM>;
+    stage_compute2<Real, N, RM, S>(va, vb, t, tw);
+    if constexpr (S + 1 < Cfg::NST) {
+      constexpr int r = Cfg::P.rad[S];
+      constexpr int L = span_before(N, S);
+      constexpr int R = Cfg::R;
+      constexpr int NB = R / r;
+      constexpr int r2 = Cfg::P.rad[S + 1];
+      constexpr int NB2 = R / r2;
+      lds_permute2<Real, N, RM, NB, r, NB2, r2>(
+          va, vb, la, lb,
+          [t](int q, int j) {
+            const int beta = t * NB + q;
+            return (beta / L) * L * r + beta % L + j * L;
+          },
+          [t](int q, int j) { return t * NB2 + q + j * (N / r2); });
+      StageLoop2<Real, N, RM, S + 1>::run(va, vb, t, tw, la, lb);
+    }
+  }
+};
+
+template <typename Real, int N, int RM, bool NT>
+__device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
+  using C = cplx_t<Real>;
+  using Cfg = FastCfg<Real, N, RM>;
+  typedef Real V4 __attribute__((ext_vector_type(4)));
+  constexpr int R = Cfg::R, T = Cfg::T;
+  constexpr size_t LB = line_lds_bytes<Real, N, RM, true>();
+  const int P2 = B >> 1;
+  const int tid = threadIdx.x;
+  const int b = tid % P2, t = tid / P2;
+  const int32_t la = blk * B + 2 * b;  // lines la, la + 1 (nlines is even)
+  const bool valid = la < p.nlines && !padding_line<Real>(p, la);
+  const double2* tw = reinterpret_cast<const double2*>(p.tw);
+  Real* ba = reinterpret_cast<Real*>(lds + (2 * b) * LB);
+  Real* bb = reinterpret_cast<Real*>(lds + (2 * b + 1) * LB);
+  LineIO<Real, NT> io;
+  if (valid) io = make_line_io<Real, NT>(p, la);
+
+  C va[R], vb[R];
+  constexpr int r0 = Cfg::P.rad[0];
+  constexpr int NB0 = R / r0;
+#pragma unroll
+  for (int q = 0; q < NB0; ++q) {
+    const C* src = reinterpret_cast<const C*>(io.ib) + static_cast<int64_t>(t * NB0 + q) * io.is;
+    const int64_t step = static_cast<int64_t>(N / r0) * io.is;
+#pragma unroll
+    for (int j = 0; j < r0; ++j) {
+      const V4 x = valid ? gload<NT>(reinterpret_cast<const V4*>(src)) : V4{Real(0), Real(0), Real(0), Real(0)};
+      const C a{x.x, x.y}, c{x.z, x.w};
+      va[q * r0 + j] = p.inverse ? cconj(a) : a;
+      vb[q * r0 + j] = p.inverse ? cconj(c) : c;
+      src += step;
+    }
+  }
+  StageLoop2<Real, N, RM, 0>::run(va, vb, t, tw, ba, bb);
+  constexpr int rl = Cfg::P.rad[Cfg::NST - 1];
+  if constexpr (rl != R) {
+    constexpr int NBl = R / rl;
+    lds_permute2<Real, N, RM, NBl, rl, 1, R>(
+        va, vb, ba, bb, [t](int q, int j) { return t * NBl + q + j * (N / rl); },
+        [t](int, int j) { return t + j * T; });
+  }
+  if (valid) {
+    C* dst = reinterpret_cast<C*>(io.ob) + static_cast<int64_t>(t) * io.os;
+    const int64_t step = static_cast<int64_t>(T) * io.os;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      C a = p.inverse ? cconj(va[j]) : va[j];
+      C c = p.inverse ? cconj(vb[j]) : vb[j];
+      V4 x;
+      x.x = a.x * io.scale;
+      x.y = a.y * io.scale;
+      x.z = c.x * io.scale;
+      x.w = c.y * io.scale;
+      gstore<NT>(reinterpret_cast<V4*>(dst), x);
+      dst += step;
+    }
+  }
+}
+
 // One block's share of a line pass: lines [blk*B, blk*B + B) of p, T = N/R
 // lanes per line.  Lanes beyond B*T (a block shape shared by several line
 // lengths) run the stages on zeros in a spare LDS slot (index B) so every
@@ -809,6 +966,13 @@ __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   fast_line_body<Real, N, RM, NT, SPLIT>(p, blk, p.lines_per_block, smem_raw);
 }
 
+template <typename Real, int N, int RM, bool NT>
+__global__ void __launch_bounds__(1024) fft_line2_fast(const LinePass p) {
+  extern __shared__ unsigned char smem_raw[];
+  const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
+  fast_line2_body<Real, N, RM, NT>(p, blk, p.lines_per_block, smem_raw);
+}
+
 // ---------------------------------------------------------------------------
 // Grouped line pass: the same pass of many independent transforms (lengths
 // from kGroupSizes, any mix) in one launch.  Virtual blocks are numbered
@@ -1018,11 +1182,79 @@ const void* fast_variant_fn(bool split, bool pair, bool nt) {
   return nt ? FastVariant<Real, N, RM, true, false>::fn(pair) : FastVariant<Real, N, RM, false, false>::fn(pair);
 }
 
+// Two lines per lane applies to FP32 C2C lanes-across-lines passes of
+// power-of-two lengths >= 128 whose consecutive lines are adjacent elements
+// of one row on both sides (the fastest batch axis is the innermost storage
+// axis, not split over ranks, with even row lengths and 16-byte aligned rank
+// buffers, so every line pair is one aligned 16-byte element pair), with an
+// even number of lines per block, of lines, of the first line and of the
+// unpadded row length.
+template <typename Real, int N> constexpr bool dual_kernel_exists() {
+  return std::is_same<Real, float>::value && N >= 128 && (N & (N - 1)) == 0;
+}
+
+inline bool dual_lines_apply(const LinePass& p, int B) {
+  if (!p.dual_lines || p.kind != kC2C || !p.lines_fast || p.nbatch < 1 || B < 2 || B % 2) return false;
+  const int j = p.nbatch - 1, fa = p.batch_axis[j];
+  if (p.batch_ext[j] % 2 || p.line_begin % 2 || p.nlines % 2 || (p.pad_real > 0 && p.pad_real % 2)) return false;
+  for (const DDesc* d : {&p.in, &p.out}) {
+    const int inner = d->layout_left ? d->map[0] : d->map[d->ndim - 1];
+    if (inner != fa || d->ntopo[fa] != 1) return false;
+    const int32_t row = (!d->layout_left && d->pitch > 0) ? d->pitch : d->ext[fa];
+    if (row % 2) return false;
+    int ranks = 1;
+    for (int a = 0; a < d->ndim; ++a) ranks *= d->ntopo[a];
+    for (int r = 0; r < ranks && r < kMaxRanks; ++r)
+      if (reinterpret_cast<uintptr_t>(d->ptr[r]) % 16 != 0) return false;
+  }
+  return true;
+}
+
+template <typename Real, int N, int RM>
+hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
+  if constexpr (!dual_kernel_exists<Real, N>()) {
+    return hipErrorInvalidConfiguration;
+  } else {
+    using Cfg = FastCfg<Real, N, RM>;
+    constexpr int T = Cfg::T;
+    const int P2 = B / 2;
+    const size_t smem = static_cast<size_t>(B) * line_lds_bytes<Real, N, RM, true>();
+    if (P2 * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
+    p.lines_per_block = B;
+    static size_t lds_limit[2] = {64 * 1024, 64 * 1024};
+    const int nt = p.nontemporal ? 1 : 0;
+    if (smem > lds_limit[nt]) {
+      const void* fn = nt ? reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true>)
+                          : reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false>);
+      hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
+      if (e != hipSuccess) return e;
+      lds_limit[nt] = smem;
+    }
+    const unsigned grid = static_cast<unsigned>((static_cast<int64_t>(p.nlines) + B - 1) / B);
+    if (grid == 0) return hipSuccess;
+    if (p.swizzle) {
+      const int32_t inner = p.batch_ext[p.nbatch - 1];
+      const int32_t C = (inner > 0 && inner % B == 0) ? inner / B : 0;
+      p.swizzle = (C > 1 && grid % static_cast<unsigned>(C) == 0 && grid / C > 1) ? C : 0;
+    }
+    if (nt) {
+      hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true>), dim3(grid), dim3(P2 * T), smem, s, p);
+    } else {
+      hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false>), dim3(grid), dim3(P2 * T), smem, s, p);
+    }
+    return hipGetLastError();
+  }
+}
+
 template <typename Real, int N, int RM>
 hipError_t launch_fast(LinePass p, hipStream_t s) {
   using Cfg = FastCfg<Real, N, RM>;
   using C = cplx_t<Real>;
   constexpr int T = Cfg::T;
+  if constexpr (dual_kernel_exists<Real, N>()) {
+    const int Bd = p.lines_per_block > 0 ? p.lines_per_block : 32;
+    if (dual_lines_apply(p, Bd)) return launch_dual<Real, N, RM>(p, Bd, s);
+  }
   const bool split = split_available<Real, N>() && p.split_lds != 0;
   const size_t line_bytes = split ? line_lds_bytes<Real, N, RM, true>() : line_lds_bytes<Real, N, RM, false>();
   int B = p.lines_per_block;

@@ -54,6 +54,10 @@ struct LinePass {
   // reals, halving the LDS of a line (more resident blocks per CU for a few
   // more barriers)
   int32_t split_lds;
+  // 1 (FP32 C2C lanes-across-lines passes whose consecutive lines are
+  // adjacent elements of one row): two lines per lane, one 16-byte access
+  // per element pair; ignored where the lines are not adjacent
+  int32_t dual_lines;
   // Weight of the interior half-spectrum entries (0 < k < n/2) of an R2C /
   // C2R line, applied to the values a C2R line loads or an R2C line stores;
   // 0 means 1.  Adjoint transforms use 1/2 (C2R) and 2 (R2C).

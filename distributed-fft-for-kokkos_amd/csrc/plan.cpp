@@ -343,6 +343,7 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   p.xcd_remap = s.xcd;
   p.swizzle = s.swz;
   p.split_lds = s.split;
+  p.dual_lines = s.dual;
   p.tw = twiddles(s.n);
   p.lines_per_block = s.lines_per_block;
   p.values_per_thread = s.values_per_thread;
@@ -586,6 +587,7 @@ void Plan::set_option(const std::string& key, long value) {
   else if (field == "xcd") passes[i].xcd = static_cast<int>(value);
   else if (field == "swz") passes[i].swz = static_cast<int>(value);
   else if (field == "split") passes[i].split = static_cast<int>(value);
+  else if (field == "dual") passes[i].dual = static_cast<int>(value);
   else throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -607,6 +609,7 @@ long Plan::get_option(const std::string& key) const {
   if (field == "xcd") return passes[i].xcd;
   if (field == "swz") return passes[i].swz;
   if (field == "split") return passes[i].split;
+  if (field == "dual") return passes[i].dual;
   throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -696,14 +699,22 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
         for (int lpb : lpbs) {
           // the column-major block walk ("swz") is left out of the search: it
           // measured slower on every pass shape tried (512^3 x pass +36 %)
-          for (int lfnt = 0; lfnt < (passes[i].kind == kC2C ? 8 : 16); ++lfnt) {
+          // bit 3: two real lines per complex FFT (R2C / C2R) or two adjacent
+          // lines per lane (FP32 C2C lanes across lines)
+          const bool c2c = passes[i].kind == kC2C;
+          for (int lfnt = 0; lfnt < (c2c && fp64_ ? 8 : 16); ++lfnt) {
+            if (c2c && (lfnt >> 3) && !(lfnt & 1)) continue;  // dual lines need lanes across lines
             PassSpec s = passes[i];
             s.values_per_thread = vpt;
             s.lines_per_block = lpb;
             s.lines_fast = lfnt & 1;
             s.nontemporal = (lfnt >> 1) & 1;
             s.xcd = (lfnt >> 2) & 1;
-            s.paired = lfnt >> 3;
+            if (c2c) {
+              s.dual = lfnt >> 3;
+            } else {
+              s.paired = lfnt >> 3;
+            }
             const float ms = time_spec(s);
             if (ms >= 0.f) timed.emplace_back(ms, s);
           }

@@ -28,6 +28,7 @@ struct PassSpec {
   int xcd = 0;           // XCD-contiguous block numbering
   int swz = 0;           // column-major walk of the block grid
   int split = 1;         // exchanges in real / imaginary halves (half the LDS; FP64, FP32 n >= 1024)
+  int dual = 0;          // FP32 C2C: two adjacent lines per lane (16-byte accesses)
 };
 
 // device table exp(-2 pi i e / n), e in [0, n): evaluated in long double,
@@ -50,11 +51,11 @@ class Plan {
   int num_passes(int direction) const;
   // tuning knobs: "chunk" (planes per chunk of the first two passes, 0 = off),
   // "pad" (scratch rows padded to 128-byte multiples: -1 auto, 0 off, 1 on),
-  // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd|swz|split", "bwd.<i>.…" (lines per
-  // block, values per thread, lanes-across-lines, nontemporal accesses, two
-  // real lines per complex FFT in an R2C / C2R pass, XCD-contiguous blocks,
-  // column-major block walk, FP64 LDS exchanges in real/imaginary halves) of
-  // pass i
+  // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd|swz|split|dual", "bwd.<i>.…" (lines
+  // per block, values per thread, lanes-across-lines, nontemporal accesses,
+  // two real lines per complex FFT in an R2C / C2R pass, XCD-contiguous
+  // blocks, column-major block walk, LDS exchanges in real/imaginary halves,
+  // two adjacent FP32 lines per lane) of pass i
   void set_option(const std::string& key, long value);
   long get_option(const std::string& key) const;
   // measure candidate launch shapes and chunk sizes on the caller's arrays

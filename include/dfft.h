@@ -151,9 +151,11 @@ int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int d
  * per thread (4, 8, 16), lane order, nontemporal global accesses (-1 = on
  * for arrays of at least 1 GiB), two real lines per complex FFT in an R2C /
  * C2R pass (default on), XCD-contiguous block numbering, a column-major walk
- * of the block grid, and radix-stage exchanges through LDS in real and
+ * of the block grid, radix-stage exchanges through LDS in real and
  * imaginary halves (default on; half the LDS per line; FP64 lines and FP32
- * lines of 1024 points or more) of pass i. */
+ * lines of 1024 points or more), and "fwd.<i>.dual" = two adjacent FP32
+ * lines per lane with 16-byte accesses (C2C lanes-across-lines passes whose
+ * consecutive lines are adjacent row elements; elsewhere ignored) of pass i. */
 int dfft_plan_set_option(dfft_plan* plan, const char* key, long value);
 int dfft_plan_get_option(const dfft_plan* plan, const char* key, long* value);
 /* Autotuner: times every launch shape of every pass and the chunk sizes on the

@@ -249,3 +249,23 @@ def test_lanes_across_lines_blocks(dev, fp64, pad):
         _check(dev, [16, 24, 40], [0, 1, 2], [2, 1, 1], [1, 2, 1], False, fp64, seed=21, configure=cfg)
         _check(dev, [32, 12, 20], [0, 1, 2], [1, 1, 2], [2, 1, 1], True, fp64, seed=22, configure=cfg)
         _check(dev, [16, 32, 9], [0, 1, 2], [2, 2, 1], [1, 2, 2], False, fp64, seed=23, configure=cfg)
+
+
+@pytest.mark.parametrize("vpt", [4, 8, 16])
+def test_dual_line_lanes_fp32(dev, vpt):
+    """FP32 C2C lanes-across-lines passes with two adjacent lines per lane
+    (forced on, 2..32 lines per block) against the oracle; shapes whose lines
+    are not adjacent pairs (odd rows) run the one-line kernels instead."""
+    for lpb in (2, 4, 8, 16, 32):
+        def cfg(plan, ins, outs, lpb=lpb):
+            for d in ("fwd", "bwd"):
+                for i in range(plan.num_passes(0 if d == "fwd" else 1)):
+                    plan.set_option(f"{d}.{i}.dual", 1)
+                    plan.set_option(f"{d}.{i}.lf", 1)
+                    plan.set_option(f"{d}.{i}.lpb", lpb)
+                    plan.set_option(f"{d}.{i}.vpt", vpt)
+        _check(dev, [128, 256, 16], [0, 1, 2], [2, 1, 1], [1, 2, 1], False, False, seed=41, configure=cfg)
+        _check(dev, [512, 8, 6], [0, 1, 2], [2, 1, 1], [1, 2, 1], False, False, seed=42, configure=cfg)
+        _check(dev, [128, 128, 10], [0, 1, 2], [2, 1, 1], [1, 2, 1], True, False, seed=43, configure=cfg)
+        _check(dev, [256, 6, 7], [0, 1, 2], [2, 1, 1], [1, 2, 1], False, False, seed=44, configure=cfg)
+        _check(dev, [256, 128, 8], [0, 1, 2], [2, 2, 1], [1, 2, 2], False, False, seed=45, configure=cfg)
