@@ -65,6 +65,11 @@ NavierStokes::NavierStokes(int n, int px, int py, double lx, double nu, double d
     NsBlock sb{3, {int32_t(e[1]), int32_t(e[2]), int32_t(e[3])}, {int32_t(st[1]), int32_t(st[2]), int32_t(st[3])}};
     plan_f_->local_starts(r, 0, e, st);
     NsBlock pb{3, {int32_t(e[1]), int32_t(e[2]), int32_t(e[3])}, {int32_t(st[1]), int32_t(st[2]), int32_t(st[3])}};
+    for (const NsBlock* blk : {&sb, &pb})
+      if (int64_t(blk->ext[0]) * blk->ext[1] * blk->ext[2] >= (int64_t(1) << 31))
+        throw Error(DFFT_ERROR_UNSUPPORTED, "Navier-Stokes rank block of 2^31 points or more");
+    sb.set_divisors();
+    pb.set_divisors();
     sblk_.push_back(sb);
     pblk_.push_back(pb);
   }

@@ -15,12 +15,21 @@ struct Mode {
   bool kz0;
 };
 
+// block-local (ix, iy, iz) of point m of a LayoutRight (ex, ey, ez) block
+// (32-bit multiply-shift divisions; m < 2^31)
+__device__ __forceinline__ void block_coords(const NsBlock& b, int64_t m, int& ix, int& iy, int& iz) {
+  const uint32_t mm = static_cast<uint32_t>(m);
+  const uint32_t r = b.div2.div(mm);
+  iz = static_cast<int>(mm - r * static_cast<uint32_t>(b.ext[2]));
+  const uint32_t q = b.div1.div(r);
+  iy = static_cast<int>(r - q * static_cast<uint32_t>(b.ext[1]));
+  ix = static_cast<int>(q);
+}
+
 // wavenumbers of spectral point m (LayoutRight (ex, ey, ez) block)
 __device__ __forceinline__ Mode mode_of(const NsGrid& g, const NsBlock& b, int64_t m) {
-  const int iz = static_cast<int>(m % b.ext[2]);
-  const int64_t r = m / b.ext[2];
-  const int iy = static_cast<int>(r % b.ext[1]);
-  const int ix = static_cast<int>(r / b.ext[1]);
+  int ix, iy, iz;
+  block_coords(b, m, ix, iy, iz);
   const int sx = signed_index(ix + b.start[0], g.n);
   const int sy = signed_index(iy + b.start[1], g.n);
   const int sz = iz + b.start[2];
@@ -202,10 +211,8 @@ __global__ void __launch_bounds__(256) taylor_green_kernel(const NsBlock b, doub
   const int64_t m_tot = static_cast<int64_t>(b.ext[0]) * b.ext[1] * b.ext[2];
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; m < m_tot; m += stride) {
-    const int iz = static_cast<int>(m % b.ext[2]);
-    const int64_t r = m / b.ext[2];
-    const int iy = static_cast<int>(r % b.ext[1]);
-    const int ix = static_cast<int>(r / b.ext[1]);
+    int ix, iy, iz;
+    block_coords(b, m, ix, iy, iz);
     const double x = h * (ix + b.start[0]), y = h * (iy + b.start[1]), z = h * (iz + b.start[2]);
     double sx, cx, sy, cy, sz, cz;
     sincos(x, &sx, &cx);
@@ -232,7 +239,8 @@ __global__ void __launch_bounds__(256) energy_kernel(const NsGrid g, const NsBlo
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   double acc = 0.0;
   for (int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; m < m_tot; m += stride) {
-    const int iz = static_cast<int>(m % b.ext[2]);
+    const uint32_t mm = static_cast<uint32_t>(m);
+    const int iz = static_cast<int>(mm - b.div2.div(mm) * static_cast<uint32_t>(b.ext[2]));
     const double f = (iz + b.start[2]) == 0 ? 1.0 : 2.0;
     double s = 0.0;
     for (int c = 0; c < 3; ++c) {

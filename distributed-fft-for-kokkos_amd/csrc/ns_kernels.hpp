@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_common.hpp"
+
 namespace dfft {
 
 // One virtual rank's block of a spectral (x, y, z/2+1) or physical (x, y, z)
@@ -13,6 +15,13 @@ struct NsBlock {
   int32_t nf;
   int32_t ext[3];
   int32_t start[3];
+  // divisions by ext[1] and ext[2] (set by set_divisors; block-local indices
+  // stay below 2^31)
+  FastDiv div1, div2;
+  void set_divisors() {
+    div1 = FastDiv(static_cast<uint32_t>(ext[1]));
+    div2 = FastDiv(static_cast<uint32_t>(ext[2]));
+  }
 };
 
 struct NsGrid {
