@@ -1252,7 +1252,8 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
   using C = cplx_t<Real>;
   constexpr int T = Cfg::T;
   if constexpr (dual_kernel_exists<Real, N>()) {
-    const int Bd = p.lines_per_block > 0 ? p.lines_per_block : 32;
+    // default: 32 lines (256-byte row segments) or as many as 1024 lanes hold
+    const int Bd = p.lines_per_block > 0 ? p.lines_per_block : (T <= 64 ? 32 : 2 * (1024 / T));
     if (dual_lines_apply(p, Bd)) return launch_dual<Real, N, RM>(p, Bd, s);
   }
   const bool split = split_available<Real, N>() && p.split_lds != 0;
