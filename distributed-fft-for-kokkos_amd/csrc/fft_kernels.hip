@@ -595,9 +595,12 @@ __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, 
   constexpr size_t LB = line_lds_bytes<Real, N, RM, true>();
   const int P2 = B >> 1;
   const int tid = threadIdx.x;
-  const int b = tid % P2, t = tid / P2;
+  // lanes beyond P2*T (grouped launches of 256 lanes) run the stages on zeros
+  // in two spare LDS slots (2*P2, 2*P2 + 1) and load and store nothing
+  const bool active = tid < P2 * T;
+  const int b = active ? tid % P2 : P2, t = active ? tid / P2 : 0;
   const int32_t la = blk * B + 2 * b;  // lines la, la + 1 (nlines is even)
-  const bool valid = la < p.nlines && !padding_line<Real>(p, la);
+  const bool valid = active && la < p.nlines && !padding_line<Real>(p, la);
   const double2* tw = reinterpret_cast<const double2*>(p.tw);
   Real* ba = reinterpret_cast<Real*>(lds + (2 * b) * LB);
   Real* bb = reinterpret_cast<Real*>(lds + (2 * b + 1) * LB);
@@ -980,7 +983,7 @@ __global__ void __launch_bounds__(1024) fft_line2_fast(const LinePass p) {
 // most a few blocks per CU walks them in a block-stride loop.  Every block
 // has kGroupThreads lanes; transform i uses items[i].lines_per_block lines.
 // ---------------------------------------------------------------------------
-template <typename Real, bool NT>
+template <typename Real, bool NT, bool DUAL>
 __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* __restrict__ items,
                                                                 const int32_t* __restrict__ first_block,
                                                                 int32_t count, int32_t total_blocks) {
@@ -997,15 +1000,30 @@ __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* 
     const int32_t i = lo;
     const LinePass& p = items[i];
     const int32_t blk = vb - first_block[i];
-    switch (p.n) {
-      case 16: fast_line_body<Real, 16, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-      case 32: fast_line_body<Real, 32, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-      case 64: fast_line_body<Real, 64, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-      case 96: fast_line_body<Real, 96, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-      case 128: fast_line_body<Real, 128, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-      case 160: fast_line_body<Real, 160, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-      case 256: fast_line_body<Real, 256, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-      default: break;
+    if constexpr (DUAL) {
+      // DUAL instances hold the FP32 members taken two lines per lane (their
+      // register sets are twice as large, so they are a kernel of their own)
+      if constexpr (std::is_same<Real, float>::value) {
+        switch (p.n) {
+          case 64: fast_line2_body<Real, 64, 8, NT>(p, blk, p.lines_per_block, lds); break;
+          case 96: fast_line2_body<Real, 96, 8, NT>(p, blk, p.lines_per_block, lds); break;
+          case 128: fast_line2_body<Real, 128, 8, NT>(p, blk, p.lines_per_block, lds); break;
+          case 160: fast_line2_body<Real, 160, 8, NT>(p, blk, p.lines_per_block, lds); break;
+          case 256: fast_line2_body<Real, 256, 8, NT>(p, blk, p.lines_per_block, lds); break;
+          default: break;
+        }
+      }
+    } else {
+      switch (p.n) {
+        case 16: fast_line_body<Real, 16, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+        case 32: fast_line_body<Real, 32, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+        case 64: fast_line_body<Real, 64, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+        case 96: fast_line_body<Real, 96, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+        case 128: fast_line_body<Real, 128, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+        case 160: fast_line_body<Real, 160, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+        case 256: fast_line_body<Real, 256, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+        default: break;
+      }
     }
     __syncthreads();  // the next virtual block reuses the LDS lines
   }
@@ -1193,7 +1211,7 @@ template <typename Real, int N> constexpr bool dual_kernel_exists() {
   return std::is_same<Real, float>::value && N >= 128 && (N & (N - 1)) == 0;
 }
 
-inline bool dual_lines_apply(const LinePass& p, int B) {
+bool dual_lines_apply_impl(const LinePass& p, int B) {
   if (!p.dual_lines || p.kind != kC2C || !p.lines_fast || p.nbatch < 1 || B < 2 || B % 2) return false;
   const int j = p.nbatch - 1, fa = p.batch_axis[j];
   if (p.batch_ext[j] % 2 || p.line_begin % 2 || p.nlines % 2 || (p.pad_real > 0 && p.pad_real % 2)) return false;
@@ -1254,7 +1272,7 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
   if constexpr (dual_kernel_exists<Real, N>()) {
     // default: 32 lines (256-byte row segments) or as many as 1024 lanes hold
     const int Bd = p.lines_per_block > 0 ? p.lines_per_block : (T <= 64 ? 32 : 2 * (1024 / T));
-    if (dual_lines_apply(p, Bd)) return launch_dual<Real, N, RM>(p, Bd, s);
+    if (dual_lines_apply_impl(p, Bd)) return launch_dual<Real, N, RM>(p, Bd, s);
   }
   const bool split = split_available<Real, N>() && p.split_lds != 0;
   const size_t line_bytes = split ? line_lds_bytes<Real, N, RM, true>() : line_lds_bytes<Real, N, RM, false>();
@@ -1401,37 +1419,76 @@ bool group_shape(int n, size_t cap, int* B, size_t* lds) {
 }
 }  // namespace
 
+namespace {
+// two lines per lane: pairs = lanes / T, lines = 2 * pairs, LDS for the
+// lines and the two spare slots (real / imaginary halves of each line)
+template <int N> void group_dual_shape_of(size_t cap, int* B, size_t* lds) {
+  using Cfg = FastCfg<float, N, 8>;
+  const size_t line_bytes = line_lds_bytes<float, N, 8, true>();
+  if (cap > kGroupLds) cap = kGroupLds;
+  int pairs = kGroupThreads / Cfg::T;
+  while (pairs > 1 && static_cast<size_t>(2 * pairs + 2) * line_bytes > cap) --pairs;
+  *B = 2 * pairs;
+  *lds = static_cast<size_t>(2 * pairs + 2) * line_bytes;
+}
+}  // namespace
+
+bool group_dual_shape(int n, size_t lds_cap, int* lines_per_block, size_t* lds_bytes) {
+  switch (n) {
+    case 64: group_dual_shape_of<64>(lds_cap, lines_per_block, lds_bytes); return true;
+    case 96: group_dual_shape_of<96>(lds_cap, lines_per_block, lds_bytes); return true;
+    case 128: group_dual_shape_of<128>(lds_cap, lines_per_block, lds_bytes); return true;
+    case 160: group_dual_shape_of<160>(lds_cap, lines_per_block, lds_bytes); return true;
+    case 256: group_dual_shape_of<256>(lds_cap, lines_per_block, lds_bytes); return true;
+    default: return false;
+  }
+}
+
+bool dual_lines_apply(const LinePass& p, int lines_per_block) { return dual_lines_apply_impl(p, lines_per_block); }
+
 bool group_line_shape(int n, bool fp64, size_t lds_cap, int* lines_per_block, size_t* lds_bytes) {
   return fp64 ? group_shape<double>(n, lds_cap, lines_per_block, lds_bytes)
               : group_shape<float>(n, lds_cap, lines_per_block, lds_bytes);
 }
 
-int group_grid(size_t lds_bytes, bool fp64) {
+int group_grid(size_t lds_bytes, bool fp64, bool dual) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  const void* f = fp64 ? reinterpret_cast<const void*>(&fft_group_pass<double, false>)
-                       : reinterpret_cast<const void*>(&fft_group_pass<float, false>);
+  const void* f = fp64   ? reinterpret_cast<const void*>(&fft_group_pass<double, false, false>)
+                  : dual ? reinterpret_cast<const void*>(&fft_group_pass<float, false, true>)
+                         : reinterpret_cast<const void*>(&fft_group_pass<float, false, false>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kGroupThreads, lds_bytes) != hipSuccess) return 0;
   return cus * (per_cu > 0 ? per_cu : 1);
 }
 
 hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
-                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, hipStream_t s) {
+                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, bool dual, hipStream_t s) {
   if (total_blocks <= 0 || count <= 0) return hipSuccess;
-  if (grid <= 0 || lds_bytes > kGroupLds) return hipErrorInvalidConfiguration;
+  if (grid <= 0 || lds_bytes > kGroupLds || (dual && fp64)) return hipErrorInvalidConfiguration;
   const dim3 g(static_cast<unsigned>(grid < total_blocks ? grid : total_blocks));
   const dim3 b(kGroupThreads);
   if (fp64) {
     if (nontemporal)
-      hipLaunchKernelGGL((fft_group_pass<double, true>), g, b, lds_bytes, s, items, first_block, count, total_blocks);
+      hipLaunchKernelGGL((fft_group_pass<double, true, false>), g, b, lds_bytes, s, items, first_block, count,
+                         total_blocks);
     else
-      hipLaunchKernelGGL((fft_group_pass<double, false>), g, b, lds_bytes, s, items, first_block, count, total_blocks);
+      hipLaunchKernelGGL((fft_group_pass<double, false, false>), g, b, lds_bytes, s, items, first_block, count,
+                         total_blocks);
+  } else if (dual) {
+    if (nontemporal)
+      hipLaunchKernelGGL((fft_group_pass<float, true, true>), g, b, lds_bytes, s, items, first_block, count,
+                         total_blocks);
+    else
+      hipLaunchKernelGGL((fft_group_pass<float, false, true>), g, b, lds_bytes, s, items, first_block, count,
+                         total_blocks);
   } else {
     if (nontemporal)
-      hipLaunchKernelGGL((fft_group_pass<float, true>), g, b, lds_bytes, s, items, first_block, count, total_blocks);
+      hipLaunchKernelGGL((fft_group_pass<float, true, false>), g, b, lds_bytes, s, items, first_block, count,
+                         total_blocks);
     else
-      hipLaunchKernelGGL((fft_group_pass<float, false>), g, b, lds_bytes, s, items, first_block, count, total_blocks);
+      hipLaunchKernelGGL((fft_group_pass<float, false, false>), g, b, lds_bytes, s, items, first_block, count,
+                         total_blocks);
   }
   return hipGetLastError();
 }

@@ -82,10 +82,17 @@ constexpr size_t kGroupLds = 64 * 1024;  // dynamic LDS of a grouped launch, at 
 // lines per block and dynamic LDS bytes of the grouped kernel for length n
 // with at most lds_cap bytes of LDS per block; false if n has no grouped kernel
 bool group_line_shape(int n, bool fp64, size_t lds_cap, int* lines_per_block, size_t* lds_bytes);
+// the same for FP32 lines taken two per lane (false if n has no such kernel)
+bool group_dual_shape(int n, size_t lds_cap, int* lines_per_block, size_t* lds_bytes);
+// true if two adjacent lines per lane apply to pass p with that many lines per
+// block (FP32 C2C, lanes across lines, consecutive lines adjacent row
+// elements on both sides, 16-byte aligned rank buffers)
+bool dual_lines_apply(const LinePass& p, int lines_per_block);
 // grid of the persistent grouped launch: resident blocks of the whole device
-int group_grid(size_t lds_bytes, bool fp64);
+int group_grid(size_t lds_bytes, bool fp64, bool dual);
+// dual: the launch holds only members with dual_lines set (FP32)
 hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
-                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, hipStream_t s);
+                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, bool dual, hipStream_t s);
 bool has_fast_line_kernel(int n);
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
 hipError_t launch_remap(const RemapArgs& a, int esize, hipStream_t s);

@@ -187,6 +187,14 @@ std::vector<LinePass> Group::build_pass(const std::vector<void*>& in, const std:
     std::size_t lds = 0;
     p.values_per_thread = 8;
     p.lines_per_block = group_line_shape(p.n, fp64_, lds_cap_, &B, &lds) ? B : 0;
+    if (!fp64_ && dual_ && kind == kC2C && p.lines_fast && group_dual_shape(p.n, lds_cap_, &B, &lds)) {
+      LinePass q = p;
+      q.dual_lines = 1;
+      if (dual_lines_apply(q, B)) {
+        p = q;
+        p.lines_per_block = B;
+      }
+    }
     passes.push_back(p);
   }
   return passes;
@@ -196,13 +204,14 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
   DirState& ds = dir_[dir];
   check(hipStreamSynchronize(s), "hipStreamSynchronize");
   const std::size_t np = static_cast<std::size_t>(max_passes_);
-  ds.items.resize(np, nullptr);
-  ds.first.resize(np, nullptr);
-  ds.count.assign(np, 0);
-  ds.total.assign(np, 0);
-  ds.grid.assign(np, 0);
-  ds.nt.assign(np, 0);
-  ds.lds.assign(np, 0);
+  const std::size_t ns = 2 * np;  // slots: (pass, one or two lines per lane)
+  ds.items.resize(ns, nullptr);
+  ds.first.resize(ns, nullptr);
+  ds.count.assign(ns, 0);
+  ds.total.assign(ns, 0);
+  ds.grid.assign(ns, 0);
+  ds.nt.assign(ns, 0);
+  ds.lds.assign(ns, 0);
   ds.single.assign(np, {});
   for (std::size_t k = 0; k < np; ++k) {
     std::vector<LinePass> all = build_pass(in, out, dir == 0, static_cast<int>(k));
@@ -214,41 +223,49 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
     }
     const int nt = nt_ >= 0 ? nt_ : (bytes >= 1024.0 * 1024 * 1024 ? 1 : 0);
     for (LinePass& p : all) p.nontemporal = nt;
-    ds.nt[k] = nt;
-    std::vector<LinePass> grouped;
-    std::vector<int32_t> first{0};
-    std::size_t lds = 0;
+    std::vector<LinePass> grouped[2];
+    std::vector<int32_t> first[2] = {{0}, {0}};
+    std::size_t lds[2] = {0, 0};
     for (const LinePass& p : all) {
       int B = 0;
       std::size_t l = 0;
       if (p.nlines == 0) continue;
-      if (group_line_shape(p.n, fp64_, lds_cap_, &B, &l)) {
-        grouped.push_back(p);
-        const int64_t nb = first.back() + (static_cast<int64_t>(p.nlines) + B - 1) / B;
+      const int v = p.dual_lines ? 1 : 0;
+      const bool shaped = v ? group_dual_shape(p.n, lds_cap_, &B, &l) : group_line_shape(p.n, fp64_, lds_cap_, &B, &l);
+      if (shaped) {
+        grouped[v].push_back(p);
+        const int64_t nb = first[v].back() + (static_cast<int64_t>(p.nlines) + B - 1) / B;
         if (nb >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many blocks in one grouped pass");
-        first.push_back(static_cast<int32_t>(nb));
-        lds = std::max(lds, l);
+        first[v].push_back(static_cast<int32_t>(nb));
+        lds[v] = std::max(lds[v], l);
       } else {
         LinePass q = p;
         q.lines_per_block = 0;  // launcher heuristic
         q.values_per_thread = 0;
+        q.dual_lines = 0;
         ds.single[k].push_back(q);
       }
     }
-    ds.count[k] = static_cast<int>(grouped.size());
-    ds.total[k] = first.back();
-    ds.lds[k] = lds;
-    if (grouped.empty()) continue;
-    const int resident = group_grid(lds, fp64_);
-    if (resident <= 0) throw Error(DFFT_ERROR_HIP, "occupancy query of the grouped kernel failed");
-    ds.grid[k] = grid_mult_ > 0 ? resident * grid_mult_ : first.back();
-    if (!ds.items[k]) {
-      check(hipMalloc(&ds.items[k], sizeof(LinePass) * members_.size()), "hipMalloc");
-      check(hipMalloc(&ds.first[k], sizeof(int32_t) * (members_.size() + 1)), "hipMalloc");
+    for (int v = 0; v < 2; ++v) {
+      const std::size_t slot = 2 * k + static_cast<std::size_t>(v);
+      ds.nt[slot] = nt;
+      ds.count[slot] = static_cast<int>(grouped[v].size());
+      ds.total[slot] = first[v].back();
+      ds.lds[slot] = lds[v];
+      if (grouped[v].empty()) continue;
+      const int resident = group_grid(lds[v], fp64_, v == 1);
+      if (resident <= 0) throw Error(DFFT_ERROR_HIP, "occupancy query of the grouped kernel failed");
+      ds.grid[slot] = grid_mult_ > 0 ? resident * grid_mult_ : first[v].back();
+      if (!ds.items[slot]) {
+        check(hipMalloc(&ds.items[slot], sizeof(LinePass) * members_.size()), "hipMalloc");
+        check(hipMalloc(&ds.first[slot], sizeof(int32_t) * (members_.size() + 1)), "hipMalloc");
+      }
+      check(hipMemcpy(ds.items[slot], grouped[v].data(), sizeof(LinePass) * grouped[v].size(),
+                      hipMemcpyHostToDevice),
+            "hipMemcpy");
+      check(hipMemcpy(ds.first[slot], first[v].data(), sizeof(int32_t) * first[v].size(), hipMemcpyHostToDevice),
+            "hipMemcpy");
     }
-    check(hipMemcpy(ds.items[k], grouped.data(), sizeof(LinePass) * grouped.size(), hipMemcpyHostToDevice),
-          "hipMemcpy");
-    check(hipMemcpy(ds.first[k], first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice), "hipMemcpy");
   }
   ds.in = in;
   ds.out = out;
@@ -258,16 +275,28 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
 void Group::launch_all(int dir, hipStream_t s) {
   const DirState& ds = dir_[dir];
   for (int k = 0; k < max_passes_; ++k) {
-    if (ds.count[k] > 0)
-      check(launch_group_pass(ds.items[k], ds.first[k], ds.count[k], ds.total[k], ds.grid[k], ds.lds[k], fp64_,
-                              ds.nt[k] != 0, s),
-            "grouped line pass");
+    for (int v = 0; v < 2; ++v) {
+      const int slot = 2 * k + v;
+      if (ds.count[slot] > 0)
+        check(launch_group_pass(ds.items[slot], ds.first[slot], ds.count[slot], ds.total[slot], ds.grid[slot],
+                                ds.lds[slot], fp64_, ds.nt[slot] != 0, v == 1, s),
+              "grouped line pass");
+    }
     for (const LinePass& p : ds.single[k]) check(launch_line_pass(p, fp64_, s), "line pass");
   }
 }
 
 int Group::num_launches(int direction) const {
-  (void)direction;
+  const DirState& ds = dir_[direction == DFFT_FORWARD ? 0 : 1];
+  if (ds.built) {
+    // the launches of the tables in use: grouped slots plus single passes
+    int n = 0;
+    for (int c : ds.count) n += c > 0 ? 1 : 0;
+    for (const auto& v : ds.single) n += static_cast<int>(v.size());
+    return n;
+  }
+  // before the first execute: one grouped launch per pass (two where some of
+  // its members take two lines per lane) plus the members without a kernel
   int n = 0;
   for (int k = 0; k < max_passes_; ++k) {
     bool grouped = false;
@@ -310,6 +339,9 @@ void Group::set_option(const std::string& key, long value) {
   } else if (key == "grid_mult") {
     if (value < 0 || value > 64) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "grid_mult must be in [0, 64]");
     grid_mult_ = static_cast<int>(value);
+  } else if (key == "dual") {
+    if (value < 0 || value > 1) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "dual must be 0 or 1");
+    dual_ = static_cast<int>(value);
   } else {
     throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown group option: " + key);
   }
@@ -320,6 +352,7 @@ long Group::get_option(const std::string& key) const {
   if (key == "lds_cap") return static_cast<long>(lds_cap_);
   if (key == "grid_mult") return grid_mult_;
   if (key == "nt") return nt_;
+  if (key == "dual") return dual_;
   throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown group option: " + key);
 }
 
@@ -329,15 +362,17 @@ void Group::autotune(void* const* in, void* const* out, hipStream_t s) {
   check(hipEventCreate(&e1), "hipEventCreate");
   float best = -1.0f;
   std::size_t best_cap = lds_cap_;
-  int best_mult = grid_mult_, best_nt = nt_;
+  int best_mult = grid_mult_, best_nt = nt_, best_dual = dual_;
   try {
     for (std::size_t cap : {std::size_t(65536), std::size_t(49152), std::size_t(32768), std::size_t(24576)}) {
-      for (int multnt = 0; multnt < 8; ++multnt) {
+      for (int multnt = 0; multnt < (fp64_ ? 8 : 16); ++multnt) {
         const int mult = (multnt & 3) == 3 ? 0 : (1 << (multnt & 3));
-        const int nt = multnt >> 2;
+        const int nt = (multnt >> 2) & 1;
+        const int dual = fp64_ ? dual_ : (multnt >> 3);
         set_option("lds_cap", static_cast<long>(cap));
         set_option("grid_mult", mult);
         set_option("nt", nt);
+        set_option("dual", dual);
         execute(in, out, DFFT_FORWARD, false, s);  // builds the tables (warm-up)
         execute(out, in, DFFT_BACKWARD, false, s);
         // best of two trials of three round trips: single trials are noisy
@@ -359,6 +394,7 @@ void Group::autotune(void* const* in, void* const* out, hipStream_t s) {
           best_cap = cap;
           best_mult = mult;
           best_nt = nt;
+          best_dual = dual;
         }
       }
     }
@@ -372,6 +408,7 @@ void Group::autotune(void* const* in, void* const* out, hipStream_t s) {
   set_option("lds_cap", static_cast<long>(best_cap));
   set_option("grid_mult", best_mult);
   set_option("nt", best_nt);
+  set_option("dual", best_dual);
 }
 
 void Group::execute(void* const* in, void* const* out, int direction, bool use_graph, hipStream_t s) {

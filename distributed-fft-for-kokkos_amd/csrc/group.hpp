@@ -61,6 +61,7 @@ class Group {
   std::size_t lds_cap_ = kGroupLds;
   int grid_mult_ = 1;
   int nt_ = -1;
+  int dual_ = 1;  // FP32: two adjacent lines per lane where they apply
   std::map<int, void*> tw_;
   // per direction: the pointer set the device tables were built for, the
   // tables of every pass (LinePass items + virtual-block prefix), members
@@ -68,11 +69,13 @@ class Group {
   struct DirState {
     std::vector<void*> in, out;
     bool built = false;
-    std::vector<LinePass*> items;     // device, one array per pass
+    // grouped launches: slot 2k + v holds the members of pass k taken one
+    // line per lane (v = 0) or two lines per lane (v = 1)
+    std::vector<LinePass*> items;     // device, one array per slot
     std::vector<int32_t*> first;      // device
     std::vector<int> count, total, grid, nt;
     std::vector<size_t> lds;
-    std::vector<std::vector<LinePass>> single;  // host: individually launched passes
+    std::vector<std::vector<LinePass>> single;  // host, per pass: individually launched passes
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } dir_[2];

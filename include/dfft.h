@@ -202,7 +202,9 @@ int dfft_group_execute(dfft_group* group, void* const* in, void* const* out, int
 int dfft_group_num_launches(const dfft_group* group, int direction, int* nlaunches);
 /* "lds_cap" (LDS bytes per block of the grouped kernel, 1024..65536),
  * "grid_mult" (multiples of the resident-block grid; 0 = one block per
- * virtual block) and "nt" (nontemporal accesses: -1 auto, 0, 1); the autotuner times combinations on the caller's arrays
+ * virtual block), "nt" (nontemporal accesses: -1 auto, 0, 1) and "dual"
+ * (FP32: two adjacent lines per lane on strided passes where they apply; 1
+ * default, 0 off); the autotuner times combinations on the caller's arrays
  * (contents are overwritten) and keeps the fastest. */
 int dfft_group_set_option(dfft_group* group, const char* key, long value);
 int dfft_group_get_option(const dfft_group* group, const char* key, long* value);
