@@ -125,3 +125,30 @@ def test_mixed_batch_64_transforms_96_128_160(dev):
         err = rel_l2(outs[i].cpu().numpy().astype(np.complex128), ref)
         print(f"member {i} N={sizes[i]} rel-L2 vs FP64 {err:.2e}")
         assert err <= 1e-6
+
+
+def test_group_dual_lines(dev):
+    """FP32 groups with two adjacent lines per lane on the strided passes (dual
+    on) and without (off) match the oracle; the member whose rows are odd (its
+    lines are no adjacent pairs) is grouped with the one-line kernel, so with
+    dual on each of its two strided passes adds a second grouped launch."""
+    import torch
+    import dfft
+    shapes = [[96, 128, 16], [128, 96, 16], [160, 64, 16], [128, 64, 7]]
+    axes_list = [[0, 1, 2]] * len(shapes)
+    launches = {}
+    for dual in (1, 0):
+        xs, ins, outs = _members(dev, shapes, False, False, axes_list, 30)
+        g = dfft.Group(ins, outs, axes_list)
+        g.set_option("dual", dual)
+        assert g.get_option("dual") == dual
+        g.execute(ins, outs, dfft.Direction.forward)
+        backs = [torch.full(list(t.shape), float("nan"), dtype=t.dtype, device=dev) for t in ins]
+        g.execute(outs, backs, dfft.Direction.backward)
+        torch.cuda.synchronize()
+        for x, o, b, axes in zip(xs, outs, backs, axes_list):
+            assert rel_l2(o.cpu().numpy(), F.fftn(x, axes, False)) <= 2e-6, f"dual={dual}"
+            assert rel_l2(b.cpu().numpy(), x) <= 2e-6, f"dual={dual}"
+        launches[dual] = g.num_launches(dfft.Direction.forward)
+        g.destroy()
+    assert launches[1] == launches[0] + 2
