@@ -12,970 +12,12 @@
 #include <hip/hip_runtime.h>
 
 #include "fft_kernels.hpp"
+#include "fft_line_impl.hpp"
 
 #include <cstdint>
 #include <type_traits>
 
 namespace dfft {
-
-// ---------------------------------------------------------------------------
-// Radix plans of the compile-time kernels.
-// ---------------------------------------------------------------------------
-struct FPlan {
-  int R = 1;    // complex values each thread keeps in registers
-  int nst = 0;  // number of stages
-  int rad[12] = {};
-  bool ok = true;
-};
-
-constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
-
-// Power-of-two stages first, then 3s and 5s.  R (values per thread) is the
-// least common multiple of the radices, doubled while below `rmin` and still
-// a divisor of n.  With an odd factor the power-of-two part uses radix 4 (not
-// 8) so R stays small: 160 = 4*4*2*5 holds 20 values per thread, not 40.
-constexpr FPlan make_plan(int n, int rmin = 1) {
-  FPlan p;
-  if (n <= 1) return p;
-  int m = n;
-  int odd = n;
-  while (odd % 2 == 0) odd /= 2;
-  if (odd == 1) {
-    while (m % 8 == 0) { p.rad[p.nst++] = 8; m /= 8; }
-  }
-  while (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
-  if (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
-  while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
-  while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; }
-  p.ok = (m == 1);
-  int R = 1;
-  for (int s = 0; s < p.nst; ++s) R = R / cgcd(R, p.rad[s]) * p.rad[s];
-  while (R < rmin && n % (2 * R) == 0) R *= 2;
-  p.R = R;
-  return p;
-}
-
-constexpr int span_before(int n, int s) {
-  FPlan p = make_plan(n);
-  int L = 1;
-  for (int i = 0; i < s; ++i) L *= p.rad[i];
-  return L;
-}
-
-// ---------------------------------------------------------------------------
-// Forward DFT butterflies on registers (exponent -2 pi i jk/r).
-// ---------------------------------------------------------------------------
-template <typename C> __device__ __forceinline__ void bfly2(C* a) {
-  C t = a[0];
-  a[0] = cadd(t, a[1]);
-  a[1] = csub(t, a[1]);
-}
-
-template <typename C> __device__ __forceinline__ void bfly4(C* a) {
-  C t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
-  C t2 = cadd(a[1], a[3]), t3 = cmul_mi(csub(a[1], a[3]));
-  a[0] = cadd(t0, t2);
-  a[2] = csub(t0, t2);
-  a[1] = cadd(t1, t3);
-  a[3] = csub(t1, t3);
-}
-
-template <typename C> __device__ __forceinline__ void bfly8(C* a) {
-  using R = decltype(a[0].x);
-  const R h = static_cast<R>(0.70710678118654752440);  // 1/sqrt(2)
-  C e[4] = {a[0], a[2], a[4], a[6]};
-  C o[4] = {a[1], a[3], a[5], a[7]};
-  bfly4(e);
-  bfly4(o);
-  // o[k] *= exp(-2 pi i k / 8)
-  C o1 = C{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
-  C o2 = cmul_mi(o[2]);
-  C o3 = C{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
-  a[0] = cadd(e[0], o[0]);
-  a[4] = csub(e[0], o[0]);
-  a[1] = cadd(e[1], o1);
-  a[5] = csub(e[1], o1);
-  a[2] = cadd(e[2], o2);
-  a[6] = csub(e[2], o2);
-  a[3] = cadd(e[3], o3);
-  a[7] = csub(e[3], o3);
-}
-
-template <typename C> __device__ __forceinline__ void bfly3(C* a) {
-  using R = decltype(a[0].x);
-  const R s3 = static_cast<R>(0.86602540378443864676);  // sin(2 pi / 3)
-  C s = cadd(a[1], a[2]);
-  C d = csub(a[1], a[2]);
-  C m = C{a[0].x - static_cast<R>(0.5) * s.x, a[0].y - static_cast<R>(0.5) * s.y};
-  C t = cmul_mi(C{s3 * d.x, s3 * d.y});
-  a[0] = cadd(a[0], s);
-  a[1] = cadd(m, t);
-  a[2] = csub(m, t);
-}
-
-template <typename C> __device__ __forceinline__ void bfly5(C* a) {
-  using R = decltype(a[0].x);
-  const R c1 = static_cast<R>(0.30901699437494742410);   // cos(2 pi/5)
-  const R c2 = static_cast<R>(-0.80901699437494742410);  // cos(4 pi/5)
-  const R s1 = static_cast<R>(0.95105651629515357212);   // sin(2 pi/5)
-  const R s2 = static_cast<R>(0.58778525229247312917);   // sin(4 pi/5)
-  C b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
-  C d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
-  C A1 = C{a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y};
-  C A2 = C{a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y};
-  C X1 = cmul_mi(C{s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y});
-  C X2 = cmul_mi(C{s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y});
-  a[0] = cadd(a[0], cadd(b1, b2));
-  a[1] = cadd(A1, X1);
-  a[4] = csub(A1, X1);
-  a[2] = cadd(A2, X2);
-  a[3] = csub(A2, X2);
-}
-
-template <int r, typename C> __device__ __forceinline__ void bfly(C* a) {
-  if constexpr (r == 2) bfly2(a);
-  else if constexpr (r == 3) bfly3(a);
-  else if constexpr (r == 4) bfly4(a);
-  else if constexpr (r == 5) bfly5(a);
-  else if constexpr (r == 8) bfly8(a);
-}
-
-// ---------------------------------------------------------------------------
-// Element access of a line.
-// ---------------------------------------------------------------------------
-// Global element access: nontemporal (streaming) or cached, chosen at
-// compile time (a runtime choice between the two forms of one access is
-// folded into the cached form by the compiler).
-template <bool NT, typename T> __device__ __forceinline__ T gload(const T* p) {
-  if constexpr (NT) return __builtin_nontemporal_load(p);
-  else return *p;
-}
-template <bool NT, typename T> __device__ __forceinline__ void gstore(T* p, T v) {
-  if constexpr (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
-}
-template <bool NT, typename Real> __device__ __forceinline__ cplx_t<Real> gload_c(const cplx_t<Real>* p) {
-  typedef Real V2 __attribute__((ext_vector_type(2)));
-  const V2 t = gload<NT>(reinterpret_cast<const V2*>(p));
-  return cplx_t<Real>{t.x, t.y};
-}
-template <bool NT, typename Real> __device__ __forceinline__ void gstore_c(cplx_t<Real>* p, cplx_t<Real> v) {
-  typedef Real V2 __attribute__((ext_vector_type(2)));
-  V2 t;
-  t.x = v.x;
-  t.y = v.y;
-  gstore<NT>(reinterpret_cast<V2*>(p), t);
-}
-
-template <typename Real, bool NT = false> struct LineIO {
-  using C = cplx_t<Real>;
-  const char* ib;
-  char* ob;
-  int64_t is, os;
-  int32_t kind, n, inverse;
-  Real scale;
-  Real hw;  // interior half-spectrum weight (1: none)
-
-  __device__ __forceinline__ C load(int32_t pos) const {
-    C v;
-    if (kind == kC2C) {
-      v = gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + pos * is);
-    } else if (kind == kR2C) {
-      v = C{gload<NT>(reinterpret_cast<const Real*>(ib) + pos * is), Real(0)};
-    } else {
-      const int32_t nh = n / 2 + 1;
-      if (pos < nh) {
-        v = gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + pos * is);
-      } else {
-        v = cconj(gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + (n - pos) * is));
-      }
-      if (pos == 0 || 2 * pos == n) {
-        v.y = Real(0);
-      } else {
-        v.x *= hw;
-        v.y *= hw;
-      }
-    }
-    return inverse ? cconj(v) : v;
-  }
-  __device__ __forceinline__ void store(int32_t pos, C v) const {
-    if (inverse) v = cconj(v);
-    v.x *= scale;
-    v.y *= scale;
-    if (kind == kC2C) {
-      gstore_c<NT, Real>(reinterpret_cast<C*>(ob) + pos * os, v);
-    } else if (kind == kR2C) {
-      if (pos != 0 && 2 * pos != n) {
-        v.x *= hw;
-        v.y *= hw;
-      }
-      if (2 * pos <= n) gstore_c<NT, Real>(reinterpret_cast<C*>(ob) + pos * os, v);
-    } else {
-      gstore<NT>(reinterpret_cast<Real*>(ob) + pos * os, v.x);
-    }
-  }
-};
-
-// Two real lines a, b per complex line.  R2C: z = a + i b, and after the
-// forward FFT Z the spectra are A_k = (Z_k + conj Z_{n-k}) / 2 and
-// B_k = (Z_k - conj Z_{n-k}) / (2i).  C2R: Z = A + i B over the Hermitian
-// extensions of both half spectra; the inverse transform of Z is a + i b.
-template <typename Real, bool NT = false> struct PairIO {
-  using C = cplx_t<Real>;
-  const char* ib[2];
-  char* ob[2];
-  int64_t is[2], os[2];  // per line: the two lines may lie in different rank blocks
-  int32_t kind, n;
-  Real scale;
-  Real hw;     // interior half-spectrum weight (1: none)
-  bool has_b;  // the pair's second line exists
-
-  // C2R input at position pos (C2R lines are conjugated: inverse = conj o
-  // FFT o conj) from the loaded half-spectrum entries A = X_a[src], Bv =
-  // X_b[src], src = pos below n/2+1 (lo) and n - pos above
-  __device__ __forceinline__ C c2r_value(int32_t pos, bool lo, C A, C Bv) const {
-    if (!lo) {
-      A = cconj(A);
-      Bv = cconj(Bv);
-    }
-    if (pos == 0 || 2 * pos == n) {
-      A.y = Real(0);
-      Bv.y = Real(0);
-    } else {
-      A.x *= hw;
-      A.y *= hw;
-      Bv.x *= hw;
-      Bv.y *= hw;
-    }
-    // conj(A + i B)
-    return C{A.x - Bv.y, -(A.y + Bv.x)};
-  }
-  // R2C: spectra of both lines at pos <= n/2 from Z_pos and Z_{n-pos}
-  __device__ __forceinline__ void r2c_values(int32_t pos, C zk, C zm, C& A, C& Bv) const {
-    const Real h = Real(0.5) * scale * ((pos != 0 && 2 * pos != n) ? hw : Real(1));
-    A = C{(zk.x + zm.x) * h, (zk.y - zm.y) * h};
-    Bv = C{(zk.y + zm.y) * h, (zm.x - zk.x) * h};
-  }
-};
-
-template <typename Real>
-__device__ __forceinline__ void decode_line(const LinePass& p, int32_t line, int32_t* idx) {
-#pragma unroll
-  for (int a = 0; a < kMaxDim; ++a) idx[a] = 0;
-  uint32_t rem = static_cast<uint32_t>(line + p.line_begin);
-#pragma unroll
-  for (int j = kMaxDim - 2; j >= 0; --j) {
-    if (j < p.nbatch) {
-      uint32_t q = p.batch_div[j].div(rem);
-      idx[p.batch_axis[j]] = static_cast<int32_t>(rem - q * static_cast<uint32_t>(p.batch_ext[j]));
-      rem = q;
-    }
-  }
-}
-
-// true if line `line` of a pass enumerated over a padded extent is padding
-template <typename Real>
-__device__ __forceinline__ bool padding_line(const LinePass& p, int32_t line) {
-  if (p.pad_real <= 0) return false;
-  int32_t idx[kMaxDim];
-  decode_line<Real>(p, line, idx);
-  return idx[p.pad_axis] >= p.pad_real;
-}
-
-template <typename Real, bool NT = false>
-__device__ __forceinline__ LineIO<Real, NT> make_line_io(const LinePass& p, int32_t line) {
-  using C = cplx_t<Real>;
-  int32_t idx[kMaxDim];
-  decode_line<Real>(p, line, idx);
-  LineIO<Real, NT> io;
-  const int64_t ies = p.kind == kR2C ? sizeof(Real) : sizeof(C);
-  const int64_t oes = p.kind == kC2R ? sizeof(Real) : sizeof(C);
-  io.ib = element_ptr(p.in, idx, ies, p.axis, io.is);
-  io.ob = element_ptr(p.out, idx, oes, p.axis, io.os);
-  io.kind = p.kind;
-  io.n = p.n;
-  io.inverse = p.inverse;
-  io.scale = static_cast<Real>(p.scale);
-  io.hw = p.herm_weight != 0.0 ? static_cast<Real>(p.herm_weight) : Real(1);
-  return io;
-}
-
-template <typename Real, bool NT>
-__device__ __forceinline__ PairIO<Real, NT> make_pair_io(const LinePass& p, int32_t pair) {
-  using C = cplx_t<Real>;
-  PairIO<Real, NT> io;
-  const int64_t ies = p.kind == kR2C ? sizeof(Real) : sizeof(C);
-  const int64_t oes = p.kind == kC2R ? sizeof(Real) : sizeof(C);
-  int32_t idx[kMaxDim];
-  const int32_t la = 2 * pair;
-  decode_line<Real>(p, la, idx);
-  io.ib[0] = element_ptr(p.in, idx, ies, p.axis, io.is[0]);
-  io.ob[0] = element_ptr(p.out, idx, oes, p.axis, io.os[0]);
-  io.has_b = la + 1 < p.nlines;
-  if (io.has_b) {
-    decode_line<Real>(p, la + 1, idx);
-    io.ib[1] = element_ptr(p.in, idx, ies, p.axis, io.is[1]);
-    io.ob[1] = element_ptr(p.out, idx, oes, p.axis, io.os[1]);
-  } else {
-    io.ib[1] = io.ib[0];
-    io.ob[1] = io.ob[0];
-    io.is[1] = io.is[0];
-    io.os[1] = io.os[0];
-  }
-  io.kind = p.kind;
-  io.n = p.n;
-  io.scale = static_cast<Real>(p.scale);
-  io.hw = p.herm_weight != 0.0 ? static_cast<Real>(p.herm_weight) : Real(1);
-  return io;
-}
-
-// ---------------------------------------------------------------------------
-// Compile-time-length Stockham kernel: each line is processed by T = N/R
-// threads holding R values in registers; radix stages exchange through LDS.
-// ---------------------------------------------------------------------------
-template <typename Real> struct LdsPad;
-template <> struct LdsPad<double> { static constexpr int shift = 3; };
-template <> struct LdsPad<float> { static constexpr int shift = 4; };
-
-template <typename Real, int N, int RM> struct FastCfg {
-  static constexpr FPlan P = make_plan(N, RM);
-  static constexpr int R = P.R;
-  static constexpr int T = N / R;
-  static constexpr int NST = P.nst;
-  static constexpr int LDSN = N + (N >> LdsPad<Real>::shift) + 1;
-  __device__ static __forceinline__ int lidx(int pos) { return pos + (pos >> LdsPad<Real>::shift); }
-};
-
-// Twiddles of a radix-r butterfly are the powers w^j (j = 1..r-1) of one
-// table entry w = exp(-2 pi i k / (L r)): one FP64 load, the other powers by
-// FP64 products (at most three multiplications deep), each rounded once to the
-// kernel's precision.
-template <int r>
-__device__ __forceinline__ void twiddle_powers(double2 w, double2* p) {
-  p[1] = w;
-  if constexpr (r > 2) p[2] = cmul(w, w);
-  if constexpr (r > 3) p[3] = cmul(p[2], w);
-  if constexpr (r > 4) p[4] = cmul(p[2], p[2]);
-  if constexpr (r > 5) p[5] = cmul(p[4], w);
-  if constexpr (r > 6) p[6] = cmul(p[3], p[3]);
-  if constexpr (r > 7) p[7] = cmul(p[4], p[3]);
-}
-
-template <typename C> __device__ __forceinline__ C to_prec(double2 w);
-template <> __device__ __forceinline__ double2 to_prec<double2>(double2 w) { return w; }
-template <> __device__ __forceinline__ float2 to_prec<float2>(double2 w) {
-  return float2{static_cast<float>(w.x), static_cast<float>(w.y)};
-}
-
-template <typename Real, int N, int RM, int S>
-__device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const double2* tw) {
-  using Cfg = FastCfg<Real, N, RM>;
-  using C = cplx_t<Real>;
-  constexpr int r = Cfg::P.rad[S];
-  constexpr int L = span_before(N, S);
-  constexpr int R = Cfg::R;
-  constexpr int NB = R / r;
-#pragma unroll
-  for (int q = 0; q < NB; ++q) {
-    if constexpr (L > 1) {
-      const int beta = t * NB + q;
-      const int k = beta % L;
-      constexpr int ts = N / (L * r);
-      double2 pw[8];
-      twiddle_powers<r>(tw[k * ts], pw);
-#pragma unroll
-      for (int j = 1; j < r; ++j) v[q * r + j] = cmul(v[q * r + j], to_prec<C>(pw[j]));
-    }
-    bfly<r>(v + q * r);
-  }
-}
-
-// Bytes of one line's LDS buffer: LDSN complex values, or LDSN reals when
-// the exchanges travel in two halves (SPLIT).
-template <typename Real, int N, int RM, bool SPLIT> constexpr size_t line_lds_bytes() {
-  return static_cast<size_t>(FastCfg<Real, N, RM>::LDSN) * (SPLIT ? sizeof(Real) : sizeof(cplx_t<Real>));
-}
-
-// Register <-> LDS permutation of one line: every lane stores its values
-// v[q*RW + j] at positions wpos(q, j), then loads v[q*RR + j] from positions
-// rpos(q, j) (NW*RW = NR*RR = R).  SPLIT moves the real parts first and the
-// imaginary parts second through a buffer of reals: after the first half a
-// lane's old real parts are in LDS and can be overwritten, its old imaginary
-// parts are still in registers for the second half.
-template <typename Real, int N, int RM, bool SPLIT, int NW, int RW, int NR, int RR, typename WPos, typename RPos>
-__device__ __forceinline__ void lds_permute(cplx_t<Real>* v, void* line, WPos wpos, RPos rpos) {
-  using Cfg = FastCfg<Real, N, RM>;
-  if constexpr (SPLIT) {
-    Real* lr = static_cast<Real*>(line);
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < NW; ++q) {
-#pragma unroll
-        for (int j = 0; j < RW; ++j) lr[Cfg::lidx(wpos(q, j))] = h ? v[q * RW + j].y : v[q * RW + j].x;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < NR; ++q) {
-#pragma unroll
-        for (int j = 0; j < RR; ++j) {
-          const Real x = lr[Cfg::lidx(rpos(q, j))];
-          if (h) {
-            v[q * RR + j].y = x;
-          } else {
-            v[q * RR + j].x = x;
-          }
-        }
-      }
-    }
-  } else {
-    cplx_t<Real>* lc = static_cast<cplx_t<Real>*>(line);
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NW; ++q) {
-#pragma unroll
-      for (int j = 0; j < RW; ++j) lc[Cfg::lidx(wpos(q, j))] = v[q * RW + j];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-#pragma unroll
-      for (int j = 0; j < RR; ++j) v[q * RR + j] = lc[Cfg::lidx(rpos(q, j))];
-    }
-  }
-}
-
-// write the outputs of stage S, read the inputs of stage S+1
-template <typename Real, int N, int RM, bool SPLIT, int S>
-__device__ __forceinline__ void stage_exchange(cplx_t<Real>* v, int t, void* line) {
-  using Cfg = FastCfg<Real, N, RM>;
-  constexpr int r = Cfg::P.rad[S];
-  constexpr int L = span_before(N, S);
-  constexpr int R = Cfg::R;
-  constexpr int NB = R / r;
-  constexpr int r2 = Cfg::P.rad[S + 1];
-  constexpr int NB2 = R / r2;
-  lds_permute<Real, N, RM, SPLIT, NB, r, NB2, r2>(
-      v, line,
-      [t](int q, int j) {
-        const int beta = t * NB + q;
-        return (beta / L) * L * r + beta % L + j * L;
-      },
-      [t](int q, int j) { return t * NB2 + q + j * (N / r2); });
-}
-
-// after the last stage the outputs sit at positions beta + j*(N/rl)
-// (beta = t*(R/rl) + q); regroup them to positions t + j*T
-template <typename Real, int N, int RM, bool SPLIT>
-__device__ __forceinline__ void final_regroup(cplx_t<Real>* v, int t, void* line) {
-  using Cfg = FastCfg<Real, N, RM>;
-  constexpr int R = Cfg::R, T = Cfg::T;
-  constexpr int rl = Cfg::P.rad[Cfg::NST - 1];
-  if constexpr (rl != R) {
-    constexpr int NBl = R / rl;
-    lds_permute<Real, N, RM, SPLIT, NBl, rl, 1, R>(
-        v, line, [t](int q, int j) { return t * NBl + q + j * (N / rl); },
-        [t](int, int j) { return t + j * T; });
-  }
-}
-
-template <typename Real, int N, int RM, bool SPLIT, int S> struct StageLoop {
-  __device__ static __forceinline__ void run(cplx_t<Real>* v, int t, const double2* tw, void* line) {
-    stage_compute<Real, N, RM, S>(v, t, tw);
-    if constexpr (S + 1 < FastCfg<Real, N, RM>::NST) {
-      stage_exchange<Real, N, RM, SPLIT, S>(v, t, line);
-      StageLoop<Real, N, RM, SPLIT, S + 1>::run(v, t, tw, line);
-    }
-  }
-};
-
-// ---------------------------------------------------------------------------
-// Two lines per lane (FP32 lanes-across-lines passes whose consecutive lines
-// are adjacent elements of one row): lane b of a block holds lines 2b and
-// 2b+1, one 16-byte access moves element k of both, and the two register
-// sets share twiddles and barriers.  B = 32 lines per block reads 256-byte
-// row segments with 16 lanes across the lines; the LDS exchanges always move
-// real and imaginary halves (2 lines x N + N/16 + 1 floats per lane pair).
-// ---------------------------------------------------------------------------
-template <typename Real, int N, int RM, int S>
-__device__ __forceinline__ void stage_compute2(cplx_t<Real>* va, cplx_t<Real>* vb, int t, const double2* tw) {
-  using Cfg = FastCfg<Real, N, RM>;
-  using C = cplx_t<Real>;
-  constexpr int r = Cfg::P.rad[S];
-  constexpr int L = span_before(N, S);
-  constexpr int R = Cfg::R;
-  constexpr int NB = R / r;
-#pragma unroll
-  for (int q = 0; q < NB; ++q) {
-    if constexpr (L > 1) {
-      const int beta = t * NB + q;
-      const int k = beta % L;
-      constexpr int ts = N / (L * r);
-      double2 pw[8];
-      twiddle_powers<r>(tw[k * ts], pw);
-#pragma unroll
-      for (int j = 1; j < r; ++j) {
-        const C w = to_prec<C>(pw[j]);
-        va[q * r + j] = cmul(va[q * r + j], w);
-        vb[q * r + j] = cmul(vb[q * r + j], w);
-      }
-    }
-    bfly<r>(va + q * r);
-    bfly<r>(vb + q * r);
-  }
-}
-
-// lds_permute for the two register sets of a lane (buffers la, lb of reals)
-template <typename Real, int N, int RM, int NW, int RW, int NR, int RR, typename WPos, typename RPos>
-__device__ __forceinline__ void lds_permute2(cplx_t<Real>* va, cplx_t<Real>* vb, Real* la, Real* lb, WPos wpos,
-                                             RPos rpos) {
-  using Cfg = FastCfg<Real, N, RM>;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NW; ++q) {
-#pragma unroll
-      for (int j = 0; j < RW; ++j) {
-        const int i = Cfg::lidx(wpos(q, j));
-        la[i] = h ? va[q * RW + j].y : va[q * RW + j].x;
-        lb[i] = h ? vb[q * RW + j].y : vb[q * RW + j].x;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-#pragma unroll
-      for (int j = 0; j < RR; ++j) {
-        const int i = Cfg::lidx(rpos(q, j));
-        const Real xa = la[i], xb = lb[i];
-        if (h) {
-          va[q * RR + j].y = xa;
-          vb[q * RR + j].y = xb;
-        } else {
-          va[q * RR + j].x = xa;
-          vb[q * RR + j].x = xb;
-        }
-      }
-    }
-  }
-}
-
-template <typename Real, int N, int RM, int S> struct StageLoop2 {
-  __device__ static __forceinline__ void run(cplx_t<Real>* va, cplx_t<Real>* vb, int t, const double2* tw, Real* la,
-                                             Real* lb) {
-    using Cfg = FastCfg<Real, N, RM>;
-    stage_compute2<Real, N, RM, S>(va, vb, t, tw);
-    if constexpr (S + 1 < Cfg::NST) {
-      constexpr int r = Cfg::P.rad[S];
-      constexpr int L = span_before(N, S);
-      constexpr int R = Cfg::R;
-      constexpr int NB = R / r;
-      constexpr int r2 = Cfg::P.rad[S + 1];
-      constexpr int NB2 = R / r2;
-      lds_permute2<Real, N, RM, NB, r, NB2, r2>(
-          va, vb, la, lb,
-          [t](int q, int j) {
-            const int beta = t * NB + q;
-            return (beta / L) * L * r + beta % L + j * L;
-          },
-          [t](int q, int j) { return t * NB2 + q + j * (N / r2); });
-      StageLoop2<Real, N, RM, S + 1>::run(va, vb, t, tw, la, lb);
-    }
-  }
-};
-
-template <typename Real, int N, int RM, bool NT>
-__device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
-  using C = cplx_t<Real>;
-  using Cfg = FastCfg<Real, N, RM>;
-  typedef Real V4 __attribute__((ext_vector_type(4)));
-  constexpr int R = Cfg::R, T = Cfg::T;
-  constexpr size_t LB = line_lds_bytes<Real, N, RM, true>();
-  const int P2 = B >> 1;
-  const int tid = threadIdx.x;
-  // lanes beyond P2*T (grouped launches of 256 lanes) run the stages on zeros
-  // in two spare LDS slots (2*P2, 2*P2 + 1) and load and store nothing
-  const bool active = tid < P2 * T;
-  const int b = active ? tid % P2 : P2, t = active ? tid / P2 : 0;
-  const int32_t la = blk * B + 2 * b;  // lines la, la + 1 (nlines is even)
-  const bool valid = active && la < p.nlines && !padding_line<Real>(p, la);
-  const double2* tw = reinterpret_cast<const double2*>(p.tw);
-  Real* ba = reinterpret_cast<Real*>(lds + (2 * b) * LB);
-  Real* bb = reinterpret_cast<Real*>(lds + (2 * b + 1) * LB);
-  LineIO<Real, NT> io;
-  if (valid) io = make_line_io<Real, NT>(p, la);
-
-  C va[R], vb[R];
-  constexpr int r0 = Cfg::P.rad[0];
-  constexpr int NB0 = R / r0;
-#pragma unroll
-  for (int q = 0; q < NB0; ++q) {
-    const C* src = reinterpret_cast<const C*>(io.ib) + static_cast<int64_t>(t * NB0 + q) * io.is;
-    const int64_t step = static_cast<int64_t>(N / r0) * io.is;
-#pragma unroll
-    for (int j = 0; j < r0; ++j) {
-      const V4 x = valid ? gload<NT>(reinterpret_cast<const V4*>(src)) : V4{Real(0), Real(0), Real(0), Real(0)};
-      const C a{x.x, x.y}, c{x.z, x.w};
-      va[q * r0 + j] = p.inverse ? cconj(a) : a;
-      vb[q * r0 + j] = p.inverse ? cconj(c) : c;
-      src += step;
-    }
-  }
-  StageLoop2<Real, N, RM, 0>::run(va, vb, t, tw, ba, bb);
-  constexpr int rl = Cfg::P.rad[Cfg::NST - 1];
-  if constexpr (rl != R) {
-    constexpr int NBl = R / rl;
-    lds_permute2<Real, N, RM, NBl, rl, 1, R>(
-        va, vb, ba, bb, [t](int q, int j) { return t * NBl + q + j * (N / rl); },
-        [t](int, int j) { return t + j * T; });
-  }
-  if (valid) {
-    C* dst = reinterpret_cast<C*>(io.ob) + static_cast<int64_t>(t) * io.os;
-    const int64_t step = static_cast<int64_t>(T) * io.os;
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      C a = p.inverse ? cconj(va[j]) : va[j];
-      C c = p.inverse ? cconj(vb[j]) : vb[j];
-      V4 x;
-      x.x = a.x * io.scale;
-      x.y = a.y * io.scale;
-      x.z = c.x * io.scale;
-      x.w = c.y * io.scale;
-      gstore<NT>(reinterpret_cast<V4*>(dst), x);
-      dst += step;
-    }
-  }
-}
-
-// One block's share of a line pass: lines [blk*B, blk*B + B) of p, T = N/R
-// lanes per line.  Lanes beyond B*T (a block shape shared by several line
-// lengths) run the stages on zeros in a spare LDS slot (index B) so every
-// lane reaches every barrier; they load and store nothing.
-template <typename Real, int N, int RM, bool NT, bool SPLIT>
-__device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
-  using C = cplx_t<Real>;
-  using Cfg = FastCfg<Real, N, RM>;
-  constexpr int R = Cfg::R, T = Cfg::T;
-  const int tid = threadIdx.x;
-  int b, t;
-  if (p.lines_fast) {
-    b = tid % B;
-    t = tid / B;
-  } else {
-    t = tid % T;
-    b = tid / T;
-  }
-  const bool active = tid < B * T;
-  if (!active) {
-    b = B;
-    t = 0;
-  }
-  const int32_t line = blk * B + b;
-  const double2* tw = reinterpret_cast<const double2*>(p.tw);
-  void* buf = lds + b * line_lds_bytes<Real, N, RM, SPLIT>();
-
-  bool valid;
-  LineIO<Real, NT> io;
-  if (T >= 64 && !p.lines_fast) {
-    // lanes along the line with T >= 64: each wave holds one line (or only
-    // spare lanes), so the line index is wave-uniform and the descriptor
-    // arithmetic of make_line_io runs once per wave on the scalar unit
-    const int32_t wl = __builtin_amdgcn_readfirstlane(line);
-    valid = active && wl < p.nlines && !padding_line<Real>(p, wl);
-    if (valid) io = make_line_io<Real, NT>(p, wl);
-  } else {
-    valid = active && line < p.nlines && !padding_line<Real>(p, line);
-    if (valid) io = make_line_io<Real, NT>(p, line);
-  }
-
-  C v[R];
-  constexpr int r0 = Cfg::P.rad[0];
-  constexpr int NB0 = R / r0;
-  if (p.kind == kC2C) {
-    // C2C lines are walked with pointer increments (no 64-bit product per element)
-#pragma unroll
-    for (int q = 0; q < NB0; ++q) {
-      const C* src = reinterpret_cast<const C*>(io.ib) + static_cast<int64_t>(t * NB0 + q) * io.is;
-      const int64_t step = static_cast<int64_t>(N / r0) * io.is;
-#pragma unroll
-      for (int j = 0; j < r0; ++j) {
-        C x = valid ? gload_c<NT, Real>(src) : C{Real(0), Real(0)};
-        v[q * r0 + j] = p.inverse ? cconj(x) : x;
-        src += step;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < NB0; ++q) {
-      const int beta = t * NB0 + q;
-#pragma unroll
-      for (int j = 0; j < r0; ++j) v[q * r0 + j] = valid ? io.load(beta + j * (N / r0)) : C{Real(0), Real(0)};
-    }
-  }
-  StageLoop<Real, N, RM, SPLIT, 0>::run(v, t, tw, buf);
-  final_regroup<Real, N, RM, SPLIT>(v, t, buf);
-
-  if (valid) {
-    if (p.kind == kC2C) {
-      C* dst = reinterpret_cast<C*>(io.ob) + static_cast<int64_t>(t) * io.os;
-      const int64_t step = static_cast<int64_t>(T) * io.os;
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        C x = p.inverse ? cconj(v[j]) : v[j];
-        x.x *= io.scale;
-        x.y *= io.scale;
-        gstore_c<NT, Real>(dst, x);
-        dst += step;
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < R; ++j) io.store(t + j * T, v[j]);
-    }
-  }
-}
-
-// Paired R2C / C2R: slot b of the block transforms lines 2q and 2q+1
-// (q = blk*B + b) as one complex line; same stage code as fast_line_body.
-template <typename Real, int N, int RM, bool NT, bool SPLIT>
-__device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
-  using C = cplx_t<Real>;
-  using Cfg = FastCfg<Real, N, RM>;
-  constexpr int R = Cfg::R, T = Cfg::T;
-  const int tid = threadIdx.x;
-  int b, t;
-  if (p.lines_fast) {
-    b = tid % B;
-    t = tid / B;
-  } else {
-    t = tid % T;
-    b = tid / T;
-  }
-  const bool active = tid < B * T;
-  if (!active) {
-    b = B;
-    t = 0;
-  }
-  const int32_t pair = blk * B + b;
-  const bool valid = active && 2 * pair < p.nlines;  // paired passes never enumerate padding
-  const double2* tw = reinterpret_cast<const double2*>(p.tw);
-  void* buf = lds + b * line_lds_bytes<Real, N, RM, SPLIT>();
-
-  PairIO<Real, NT> io{};
-  if (valid) io = make_pair_io<Real, NT>(p, pair);
-
-  // line elements are walked with pointer increments (positions beta + j*S
-  // on load, t + j*T on store): no 64-bit product per element
-  C v[R];
-  C mirror[R];  // C2R: input at position n - pos, formed by the lane that loads X[pos]
-  {
-    constexpr int r0 = Cfg::P.rad[0];
-    constexpr int NB0 = R / r0;
-    constexpr int S = N / r0;
-    const bool vb = valid && io.has_b;
-    const int64_t sa = static_cast<int64_t>(S) * io.is[0], sb = static_cast<int64_t>(S) * io.is[1];
-#pragma unroll
-    for (int q = 0; q < NB0; ++q) {
-      const int beta = t * NB0 + q;
-      if (p.kind == kR2C) {
-        const Real* pa = reinterpret_cast<const Real*>(io.ib[0]) + static_cast<int64_t>(beta) * io.is[0];
-        const Real* pb = reinterpret_cast<const Real*>(io.ib[1]) + static_cast<int64_t>(beta) * io.is[1];
-#pragma unroll
-        for (int j = 0; j < r0; ++j) {
-          const Real xa = valid ? gload<NT>(pa) : Real(0);
-          const Real xb = vb ? gload<NT>(pb) : Real(0);
-          v[q * r0 + j] = C{xa, xb};
-          pa += sa;
-          pb += sb;
-        }
-      } else {
-        // C2R: each half-spectrum entry X[m], m <= n/2, is loaded once, by the
-        // lane holding position m; that lane also forms the input at the
-        // mirrored position n - m (from conj X[m]) and hands it over through
-        // LDS below
-        const C* la = reinterpret_cast<const C*>(io.ib[0]) + static_cast<int64_t>(beta) * io.is[0];
-        const C* lb = reinterpret_cast<const C*>(io.ib[1]) + static_cast<int64_t>(beta) * io.is[1];
-#pragma unroll
-        for (int j = 0; j < r0; ++j) {
-          const int pos = beta + j * S;
-          if (2 * pos <= N) {
-            const C A = valid ? gload_c<NT, Real>(la) : C{Real(0), Real(0)};
-            const C Bv = vb ? gload_c<NT, Real>(lb) : C{Real(0), Real(0)};
-            v[q * r0 + j] = io.c2r_value(pos, true, A, Bv);
-            mirror[q * r0 + j] = io.c2r_value(N - pos, false, A, Bv);
-          }
-          la += sa;
-          lb += sb;
-        }
-      }
-    }
-    if (p.kind == kC2R) {
-      // positions above n/2 receive the value formed by the lane holding n - pos
-      if constexpr (SPLIT) {
-        Real* lr = static_cast<Real*>(buf);
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          if (h) __syncthreads();
-#pragma unroll
-          for (int q = 0; q < NB0; ++q) {
-#pragma unroll
-            for (int j = 0; j < r0; ++j) {
-              const int pos = t * NB0 + q + j * S;
-              if (pos > 0 && 2 * pos < N) lr[Cfg::lidx(N - pos)] = h ? mirror[q * r0 + j].y : mirror[q * r0 + j].x;
-            }
-          }
-          __syncthreads();
-#pragma unroll
-          for (int q = 0; q < NB0; ++q) {
-#pragma unroll
-            for (int j = 0; j < r0; ++j) {
-              const int pos = t * NB0 + q + j * S;
-              if (2 * pos > N) {
-                const Real x = lr[Cfg::lidx(pos)];
-                if (h) {
-                  v[q * r0 + j].y = x;
-                } else {
-                  v[q * r0 + j].x = x;
-                }
-              }
-            }
-          }
-        }
-      } else {
-        C* lc = static_cast<C*>(buf);
-#pragma unroll
-        for (int q = 0; q < NB0; ++q) {
-#pragma unroll
-          for (int j = 0; j < r0; ++j) {
-            const int pos = t * NB0 + q + j * S;
-            if (pos > 0 && 2 * pos < N) lc[Cfg::lidx(N - pos)] = mirror[q * r0 + j];
-          }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < NB0; ++q) {
-#pragma unroll
-          for (int j = 0; j < r0; ++j) {
-            const int pos = t * NB0 + q + j * S;
-            if (2 * pos > N) v[q * r0 + j] = lc[Cfg::lidx(pos)];
-          }
-        }
-      }
-    }
-  }
-  StageLoop<Real, N, RM, SPLIT, 0>::run(v, t, tw, buf);
-  final_regroup<Real, N, RM, SPLIT>(v, t, buf);
-  const int64_t ta = static_cast<int64_t>(T) * io.os[0], tb = static_cast<int64_t>(T) * io.os[1];
-  if (p.kind == kC2R) {
-    // w = FFT(conj Z) at each position, so a + i b = conj(w)
-    if (valid) {
-      Real* pa = reinterpret_cast<Real*>(io.ob[0]) + static_cast<int64_t>(t) * io.os[0];
-      Real* pb = reinterpret_cast<Real*>(io.ob[1]) + static_cast<int64_t>(t) * io.os[1];
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        gstore<NT>(pa, v[j].x * io.scale);
-        if (io.has_b) gstore<NT>(pb, -v[j].y * io.scale);
-        pa += ta;
-        pb += tb;
-      }
-    }
-    return;
-  }
-  // R2C: Z_{n-pos} lives in another lane; exchange the whole line through LDS
-  C w[R];  // w[j] = Z_{(n - pos) mod n} for pos = t + j*T
-  if constexpr (SPLIT) {
-    Real* lr = static_cast<Real*>(buf);
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < R; ++j) lr[Cfg::lidx(t + j * T)] = h ? v[j].y : v[j].x;
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const int pos = t + j * T;
-        const Real x = lr[Cfg::lidx(pos == 0 ? 0 : N - pos)];
-        if (h) {
-          w[j].y = x;
-        } else {
-          w[j].x = x;
-        }
-      }
-    }
-  } else {
-    C* lc = static_cast<C*>(buf);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < R; ++j) lc[Cfg::lidx(t + j * T)] = v[j];
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int pos = t + j * T;
-      w[j] = lc[Cfg::lidx(pos == 0 ? 0 : N - pos)];
-    }
-  }
-  if (valid) {
-    C* pa = reinterpret_cast<C*>(io.ob[0]) + static_cast<int64_t>(t) * io.os[0];
-    C* pb = reinterpret_cast<C*>(io.ob[1]) + static_cast<int64_t>(t) * io.os[1];
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int pos = t + j * T;
-      if (2 * pos <= N) {
-        C A, Bv;
-        io.r2c_values(pos, v[j], w[j], A, Bv);
-        gstore_c<NT, Real>(pa, A);
-        if (io.has_b) gstore_c<NT, Real>(pb, Bv);
-      }
-      pa += ta;
-      pb += tb;
-    }
-  }
-}
-
-// Workgroups go to the 8 XCDs round-robin; renumber them so XCD x owns the
-// x-th contiguous range of blocks (a bijection for any grid size).
-__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t nb) {
-  constexpr int32_t kXcds = 8;
-  const int32_t q = nb / kXcds, r = nb % kXcds;
-  const int32_t x = b % kXcds, i = b / kXcds;
-  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
-
-// Block of the pass for hardware block b: optional XCD-contiguous numbering,
-// then an optional column-major walk of the (nb / C) x C block grid, where C
-// blocks cover one row of the fastest batch axis (p.swizzle = C; the launcher
-// sets it only when it divides the grid).
-__device__ __forceinline__ int32_t pass_block(const LinePass& p, int32_t b, int32_t nb) {
-  if (p.xcd_remap) b = xcd_block(b, nb);
-  if (p.swizzle > 1) {
-    const int32_t rows = nb / p.swizzle;
-    b = (b % rows) * p.swizzle + b / rows;
-  }
-  return b;
-}
-
-template <typename Real, int N, int RM, bool NT, bool SPLIT>
-__global__ void __launch_bounds__(1024) fft_pair_fast(const LinePass p) {
-  extern __shared__ unsigned char smem_raw[];
-  const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
-  fast_pair_body<Real, N, RM, NT, SPLIT>(p, blk, p.lines_per_block, smem_raw);
-}
-
-template <typename Real, int N, int RM, bool NT, bool SPLIT>
-__global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
-  extern __shared__ unsigned char smem_raw[];
-  const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
-  fast_line_body<Real, N, RM, NT, SPLIT>(p, blk, p.lines_per_block, smem_raw);
-}
-
-template <typename Real, int N, int RM, bool NT>
-__global__ void __launch_bounds__(1024) fft_line2_fast(const LinePass p) {
-  extern __shared__ unsigned char smem_raw[];
-  const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
-  fast_line2_body<Real, N, RM, NT>(p, blk, p.lines_per_block, smem_raw);
-}
-
 // ---------------------------------------------------------------------------
 // Grouped line pass: the same pass of many independent transforms (lengths
 // from kGroupSizes, any mix) in one launch.  Virtual blocks are numbered
@@ -1149,190 +191,9 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
 }
 
 // ---------------------------------------------------------------------------
-// Launchers
+// Launchers of the generic kernel and dispatch to the fast-kernel units
 // ---------------------------------------------------------------------------
 namespace {
-
-// One compile-time variant of the fast kernels: paired or single lines,
-// nontemporal or cached accesses, split or complex LDS exchanges.
-template <typename Real, int N, int RM, bool NT, bool SPLIT> struct FastVariant {
-  static constexpr bool kPairs = !(std::is_same<Real, double>::value && (N % 3 == 0 || N % 5 == 0));
-  static const void* fn(bool pair) {
-    if constexpr (kPairs) {
-      if (pair) return reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, NT, SPLIT>);
-    }
-    return reinterpret_cast<const void*>(&fft_line_fast<Real, N, RM, NT, SPLIT>);
-  }
-  static void launch(bool pair, unsigned grid, unsigned threads, size_t smem, hipStream_t s, const LinePass& p) {
-    if constexpr (kPairs) {
-      if (pair) {
-        hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, NT, SPLIT>), dim3(grid), dim3(threads), smem, s, p);
-        return;
-      }
-    }
-    hipLaunchKernelGGL((fft_line_fast<Real, N, RM, NT, SPLIT>), dim3(grid), dim3(threads), smem, s, p);
-  }
-};
-
-// split exchanges exist for FP64, and for FP32 lines of 1024 points and more
-// (shorter FP32 lines are small in LDS already and ran slower split)
-template <typename Real, int N> constexpr bool split_available() {
-  return std::is_same<Real, double>::value || N >= 1024;
-}
-
-template <typename Real, int N, int RM, bool NT>
-void launch_fast_variant(bool split, bool pair, unsigned grid, unsigned threads, size_t smem, hipStream_t s,
-                         const LinePass& p) {
-  if constexpr (split_available<Real, N>()) {
-    if (split) {
-      FastVariant<Real, N, RM, NT, true>::launch(pair, grid, threads, smem, s, p);
-      return;
-    }
-  }
-  FastVariant<Real, N, RM, NT, false>::launch(pair, grid, threads, smem, s, p);
-}
-
-template <typename Real, int N, int RM>
-const void* fast_variant_fn(bool split, bool pair, bool nt) {
-  if constexpr (split_available<Real, N>()) {
-    if (split) return nt ? FastVariant<Real, N, RM, true, true>::fn(pair) : FastVariant<Real, N, RM, false, true>::fn(pair);
-  }
-  return nt ? FastVariant<Real, N, RM, true, false>::fn(pair) : FastVariant<Real, N, RM, false, false>::fn(pair);
-}
-
-// Two lines per lane applies to FP32 C2C lanes-across-lines passes of
-// power-of-two lengths >= 128 whose consecutive lines are adjacent elements
-// of one row on both sides (the fastest batch axis is the innermost storage
-// axis, not split over ranks, with even row lengths and 16-byte aligned rank
-// buffers, so every line pair is one aligned 16-byte element pair), with an
-// even number of lines per block, of lines, of the first line and of the
-// unpadded row length.
-template <typename Real, int N> constexpr bool dual_kernel_exists() {
-  return std::is_same<Real, float>::value && N >= 128 && (N & (N - 1)) == 0;
-}
-
-bool dual_lines_apply_impl(const LinePass& p, int B) {
-  if (!p.dual_lines || p.kind != kC2C || !p.lines_fast || p.nbatch < 1 || B < 2 || B % 2) return false;
-  const int j = p.nbatch - 1, fa = p.batch_axis[j];
-  if (p.batch_ext[j] % 2 || p.line_begin % 2 || p.nlines % 2 || (p.pad_real > 0 && p.pad_real % 2)) return false;
-  for (const DDesc* d : {&p.in, &p.out}) {
-    const int inner = d->layout_left ? d->map[0] : d->map[d->ndim - 1];
-    if (inner != fa || d->ntopo[fa] != 1) return false;
-    const int32_t row = (!d->layout_left && d->pitch > 0) ? d->pitch : d->ext[fa];
-    if (row % 2) return false;
-    int ranks = 1;
-    for (int a = 0; a < d->ndim; ++a) ranks *= d->ntopo[a];
-    for (int r = 0; r < ranks && r < kMaxRanks; ++r)
-      if (reinterpret_cast<uintptr_t>(d->ptr[r]) % 16 != 0) return false;
-  }
-  return true;
-}
-
-template <typename Real, int N, int RM>
-hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
-  if constexpr (!dual_kernel_exists<Real, N>()) {
-    return hipErrorInvalidConfiguration;
-  } else {
-    using Cfg = FastCfg<Real, N, RM>;
-    constexpr int T = Cfg::T;
-    const int P2 = B / 2;
-    const size_t smem = static_cast<size_t>(B) * line_lds_bytes<Real, N, RM, true>();
-    if (P2 * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
-    p.lines_per_block = B;
-    static size_t lds_limit[2] = {64 * 1024, 64 * 1024};
-    const int nt = p.nontemporal ? 1 : 0;
-    if (smem > lds_limit[nt]) {
-      const void* fn = nt ? reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true>)
-                          : reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false>);
-      hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
-      if (e != hipSuccess) return e;
-      lds_limit[nt] = smem;
-    }
-    const unsigned grid = static_cast<unsigned>((static_cast<int64_t>(p.nlines) + B - 1) / B);
-    if (grid == 0) return hipSuccess;
-    if (p.swizzle) {
-      const int32_t inner = p.batch_ext[p.nbatch - 1];
-      const int32_t C = (inner > 0 && inner % B == 0) ? inner / B : 0;
-      p.swizzle = (C > 1 && grid % static_cast<unsigned>(C) == 0 && grid / C > 1) ? C : 0;
-    }
-    if (nt) {
-      hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true>), dim3(grid), dim3(P2 * T), smem, s, p);
-    } else {
-      hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false>), dim3(grid), dim3(P2 * T), smem, s, p);
-    }
-    return hipGetLastError();
-  }
-}
-
-template <typename Real, int N, int RM>
-hipError_t launch_fast(LinePass p, hipStream_t s) {
-  using Cfg = FastCfg<Real, N, RM>;
-  using C = cplx_t<Real>;
-  constexpr int T = Cfg::T;
-  if constexpr (dual_kernel_exists<Real, N>()) {
-    // default: 32 lines (256-byte row segments) or as many as 1024 lanes hold
-    const int Bd = p.lines_per_block > 0 ? p.lines_per_block : (T <= 64 ? 32 : 2 * (1024 / T));
-    if (dual_lines_apply_impl(p, Bd)) return launch_dual<Real, N, RM>(p, Bd, s);
-  }
-  const bool split = split_available<Real, N>() && p.split_lds != 0;
-  const size_t line_bytes = split ? line_lds_bytes<Real, N, RM, true>() : line_lds_bytes<Real, N, RM, false>();
-  int B = p.lines_per_block;
-  if (B <= 0) {
-    if (p.lines_fast) {
-      B = 128 / static_cast<int>(sizeof(C));  // lines per 128-byte row segment
-      while (B * T > 1024 && B > 1) B /= 2;
-    } else {
-      B = T >= 256 ? 1 : 256 / T;
-    }
-    while (B > 1 && static_cast<size_t>(B) * line_bytes > 160 * 1024) B /= 2;
-  }
-  const size_t smem = static_cast<size_t>(B) * line_bytes;
-  if (B * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
-  p.lines_per_block = B;
-  // two real lines per complex FFT for forward R2C and inverse C2R passes
-  // (kernels for 8 values per thread)
-  // (not for FP64 lengths with an odd factor: the pair kernel's two sets of
-  // 20-40 complex values per lane spill registers there)
-  const bool pair = FastVariant<Real, N, RM, false, false>::kPairs && RM == 8 && p.paired &&
-                    ((p.kind == kR2C && !p.inverse) || (p.kind == kC2R && p.inverse));
-  const bool nt = p.nontemporal != 0;
-  const int v = 4 * (split ? 1 : 0) + 2 * (pair ? 1 : 0) + (nt ? 1 : 0);
-  // raise the dynamic-LDS limit once per kernel instance (never inside a
-  // stream capture that follows an eager launch of the same shape)
-  static size_t lds_limit[8] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024,
-                                64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
-  if (smem > lds_limit[v]) {
-    hipError_t e = hipFuncSetAttribute(fast_variant_fn<Real, N, RM>(split, pair, nt),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
-    if (e != hipSuccess) return e;
-    lds_limit[v] = smem;
-  }
-  const int64_t units = pair ? (static_cast<int64_t>(p.nlines) + 1) / 2 : p.nlines;
-  const unsigned grid = static_cast<unsigned>((units + B - 1) / B);
-  if (grid == 0) return hipSuccess;
-  if (p.swizzle) {
-    // C = blocks per row of the fastest batch axis; the walk needs whole rows
-    const int32_t inner = p.nbatch > 0 ? p.batch_ext[p.nbatch - 1] : 0;
-    const int32_t per = pair ? 2 * B : B;
-    const int32_t C = (inner > 0 && inner % per == 0) ? inner / per : 0;
-    p.swizzle = (C > 1 && grid % static_cast<unsigned>(C) == 0 && grid / C > 1) ? C : 0;
-  }
-  if (nt) {
-    launch_fast_variant<Real, N, RM, true>(split, pair, grid, static_cast<unsigned>(T * B), smem, s, p);
-  } else {
-    launch_fast_variant<Real, N, RM, false>(split, pair, grid, static_cast<unsigned>(T * B), smem, s, p);
-  }
-  return hipGetLastError();
-}
-
-template <typename Real, int N>
-hipError_t launch_fast_r(const LinePass& p, hipStream_t s) {
-  switch (p.values_per_thread) {
-    case 4: return launch_fast<Real, N, 4>(p, s);
-    case 16: return launch_fast<Real, N, 16>(p, s);
-    default: return launch_fast<Real, N, 8>(p, s);
-  }
-}
 
 template <typename Real>
 hipError_t launch_generic(LinePass p, hipStream_t s) {
@@ -1371,22 +232,9 @@ hipError_t launch_generic(LinePass p, hipStream_t s) {
 
 template <typename Real>
 hipError_t dispatch(const LinePass& p, hipStream_t s) {
-  switch (p.n) {
-    case 2: return launch_fast_r<Real, 2>(p, s);
-    case 4: return launch_fast_r<Real, 4>(p, s);
-    case 8: return launch_fast_r<Real, 8>(p, s);
-    case 16: return launch_fast_r<Real, 16>(p, s);
-    case 32: return launch_fast_r<Real, 32>(p, s);
-    case 64: return launch_fast_r<Real, 64>(p, s);
-    case 96: return launch_fast_r<Real, 96>(p, s);
-    case 128: return launch_fast_r<Real, 128>(p, s);
-    case 160: return launch_fast_r<Real, 160>(p, s);
-    case 256: return launch_fast_r<Real, 256>(p, s);
-    case 512: return launch_fast_r<Real, 512>(p, s);
-    case 1024: return launch_fast_r<Real, 1024>(p, s);
-    case 2048: return launch_fast_r<Real, 2048>(p, s);
-    default: return launch_generic<Real>(p, s);
-  }
+  const bool fp64 = std::is_same<Real, double>::value;
+  if (p.n <= 160) return fp64 ? dispatch_fast_f64_short(p, s) : dispatch_fast_f32_short(p, s);
+  return fp64 ? dispatch_fast_f64_long(p, s) : dispatch_fast_f32_long(p, s);
 }
 
 }  // namespace

@@ -34,22 +34,32 @@ def _sections(elf):
     return [(nm(n), typ, off, size) for n, typ, off, size in raw]
 
 
-def _code_object(lib_bytes, arch="gfx950"):
+def _code_objects(lib_bytes, arch="gfx950"):
+    """The gfx950 code objects of every offload bundle in .hip_fatbin (one
+    bundle per HIP translation unit)."""
     fat = [s for s in _sections(lib_bytes) if s[0] == ".hip_fatbin"]
     assert fat, "libdfft.so has no .hip_fatbin section"
     _, _, off, size = fat[0]
     blob = lib_bytes[off: off + size]
     magic = b"__CLANG_OFFLOAD_BUNDLE__"
     assert blob.startswith(magic), "unexpected offload bundle format"
-    (count,) = struct.unpack_from("<Q", blob, len(magic))
-    pos = len(magic) + 8
-    for _ in range(count):
-        eoff, esize, tlen = struct.unpack_from("<QQQ", blob, pos)
-        triple = blob[pos + 24: pos + 24 + tlen].decode()
-        pos += 24 + tlen
-        if triple.endswith(arch):
-            return blob[eoff: eoff + esize]
-    raise AssertionError(f"no {arch} code object in the bundle")
+    found = []
+    start = 0
+    while True:
+        start = blob.find(magic, start)
+        if start < 0:
+            break
+        (count,) = struct.unpack_from("<Q", blob, start + len(magic))
+        pos = start + len(magic) + 8
+        for _ in range(count):
+            eoff, esize, tlen = struct.unpack_from("<QQQ", blob, pos)
+            triple = blob[pos + 24: pos + 24 + tlen].decode()
+            pos += 24 + tlen
+            if triple.endswith(arch):
+                found.append(blob[start + eoff: start + eoff + esize])
+        start += len(magic)
+    assert found, f"no {arch} code object in the bundles"
+    return found
 
 
 def _kernels(co):
@@ -72,11 +82,14 @@ def _kernels(co):
 def kernels():
     if not LIB.exists():
         pytest.skip("libdfft.so is not built")
-    return _kernels(_code_object(LIB.read_bytes()))
+    return [k for co in _code_objects(LIB.read_bytes()) for k in _kernels(co)]
 
 
 def test_metadata_lists_the_line_kernels(kernels):
+    """Every kernel family is present, and each kernel is instantiated in one
+    translation unit only (the line kernels are spread over several)."""
     names = [k[".name"] for k in kernels]
+    assert len(names) == len(set(names)), "a kernel is instantiated in more than one unit"
     assert any("fft_line_fast" in n for n in names)
     assert any("fft_pair_fast" in n for n in names)
     assert any("fft_group_pass" in n for n in names)
