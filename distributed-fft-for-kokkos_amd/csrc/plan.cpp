@@ -677,9 +677,9 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
       if (!has_fast_line_kernel(passes[i].n)) continue;
       DDesc a = i == 0 ? src : scr;
       DDesc b = i + 1 == passes.size() ? dst : scr;
-      // one timing of 3 launches per shape, then the four fastest shapes are
-      // timed twice more and the best single timing wins (one noisy sample
-      // does not decide)
+      // one timing of 3 launches per shape, then the six fastest shapes are
+      // timed three more times and the best single timing wins (one noisy
+      // sample does not decide)
       auto time_spec = [&](const PassSpec& s) -> float {
         LinePass p = make_pass(s, a, b, !forward, 1.0);
         if (launch_line_pass(p, fp64_, stream) != hipSuccess) {
@@ -723,7 +723,7 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
       if (timed.empty()) continue;
       std::stable_sort(timed.begin(), timed.end(),
                        [](const auto& x, const auto& y) { return x.first < y.first; });
-      const std::size_t finalists = std::min<std::size_t>(4, timed.size());
+      const std::size_t finalists = std::min<std::size_t>(6, timed.size());
       float best = 1e30f;
       PassSpec best_spec = passes[i];
       for (std::size_t f = 0; f < finalists; ++f) {
@@ -733,7 +733,7 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
           PassSpec s = timed[f].second;
           if (flip) s.split = s.split ? 0 : 1;
           float ms = flip ? 1e30f : timed[f].first;
-          for (int trial = 0; trial < (flip ? 3 : 2); ++trial) {
+          for (int trial = 0; trial < 3; ++trial) {
             const float m = time_spec(s);
             if (m >= 0.f) ms = std::min(ms, m);
           }
