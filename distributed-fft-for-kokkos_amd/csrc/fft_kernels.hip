@@ -1,6 +1,8 @@
-// HIP kernels for gfx950 (CDNA4): batched 1-D Stockham FFT passes over
-// distributed arrays, the exchange remap, and the reference-exact pack /
-// unpack / all-to-all kernels.
+// HIP kernels for gfx950 (CDNA4): the grouped and runtime-length line passes,
+// the line-pass launch entry (the compile-time-length Stockham kernels are
+// templates in fft_line_impl.hpp, instantiated in the fft_lines_*.hip units),
+// the exchange remap, and the reference-exact pack / unpack / all-to-all
+// kernels.
 //
 // FFT conventions (the ones kokkos-fft exposes to the reference plans,
 // KokkosFFT_Distributed_SlabPlan.hpp:978-1081): forward X[k] = sum_j x[j]

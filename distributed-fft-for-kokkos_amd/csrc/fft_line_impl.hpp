@@ -1,6 +1,12 @@
 // Templates of the compile-time-length line-pass kernels and their
 // launchers, shared by the translation units that instantiate them (one per
 // precision and range of lengths, compiled in parallel).
+//
+// FFT conventions (the ones kokkos-fft exposes to the reference plans,
+// KokkosFFT_Distributed_SlabPlan.hpp:978-1081): forward X[k] = sum_j x[j]
+// exp(-2 pi i jk/n), backward with +i, both unscaled; scaling is applied by
+// the caller-supplied factor at the final store.  Inverse transforms run the
+// forward kernel between two conjugations (IDFT(x) = conj(DFT(conj(x)))).
 #pragma once
 
 #include <hip/hip_runtime.h>
