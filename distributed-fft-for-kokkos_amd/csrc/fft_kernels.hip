@@ -218,12 +218,10 @@ hipError_t launch_generic(LinePass p, hipStream_t s) {
   while (B > 1 && static_cast<size_t>(2) * B * p.n * sizeof(C) > 64 * 1024) B /= 2;
   const size_t smem = static_cast<size_t>(2) * B * p.n * sizeof(C);
   if (smem > 160 * 1024) return hipErrorInvalidValue;
-  static size_t lds_limit = 64 * 1024;
-  if (smem > lds_limit) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_line_generic<Real>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
+  static std::atomic<size_t> lds_limit{64 * 1024};
+  {
+    const hipError_t e = raise_lds_limit(lds_limit, reinterpret_cast<const void*>(&fft_line_generic<Real>), smem);
     if (e != hipSuccess) return e;
-    lds_limit = smem;
   }
   p.lines_per_block = B;
   const unsigned grid = static_cast<unsigned>((p.nlines + B - 1) / B);

@@ -13,7 +13,9 @@
 
 #include "fft_kernels.hpp"
 
+#include <atomic>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 namespace dfft {
@@ -981,6 +983,22 @@ __global__ void __launch_bounds__(1024) fft_line2_fast(const LinePass p) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// Raises kernel fn's dynamic-LDS limit to at least `bytes` (the cached
+// limit belongs to that one kernel; updates only grow it and are serialised,
+// so concurrent launches from several host threads never lower it).  Never
+// called inside a stream capture that follows an eager launch of the same
+// shape: the limit is then already raised.
+inline hipError_t raise_lds_limit(std::atomic<size_t>& limit, const void* fn, size_t bytes) {
+  if (bytes <= limit.load(std::memory_order_acquire)) return hipSuccess;
+  static std::mutex m;
+  std::lock_guard<std::mutex> lock(m);
+  if (bytes <= limit.load(std::memory_order_relaxed)) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+  if (e != hipSuccess) return e;
+  limit.store(bytes, std::memory_order_release);
+  return hipSuccess;
+}
+
 // One compile-time variant of the fast kernels: paired or single lines,
 // nontemporal or cached accesses, split or complex LDS exchanges.
 template <typename Real, int N, int RM, bool NT, bool SPLIT> struct FastVariant {
@@ -1067,15 +1085,12 @@ hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
     const size_t smem = static_cast<size_t>(B) * line_lds_bytes<Real, N, RM, true>();
     if (P2 * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
     p.lines_per_block = B;
-    static size_t lds_limit[2] = {64 * 1024, 64 * 1024};
+    static std::atomic<size_t> lds_limit[2] = {64 * 1024, 64 * 1024};
     const int nt = p.nontemporal ? 1 : 0;
-    if (smem > lds_limit[nt]) {
-      const void* fn = nt ? reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true>)
-                          : reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false>);
-      hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
-      if (e != hipSuccess) return e;
-      lds_limit[nt] = smem;
-    }
+    const void* fn = nt ? reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true>)
+                        : reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false>);
+    const hipError_t e = raise_lds_limit(lds_limit[nt], fn, smem);
+    if (e != hipSuccess) return e;
     const unsigned grid = static_cast<unsigned>((static_cast<int64_t>(p.nlines) + B - 1) / B);
     if (grid == 0) return hipSuccess;
     if (p.swizzle) {
@@ -1125,15 +1140,12 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
                     ((p.kind == kR2C && !p.inverse) || (p.kind == kC2R && p.inverse));
   const bool nt = p.nontemporal != 0;
   const int v = 4 * (split ? 1 : 0) + 2 * (pair ? 1 : 0) + (nt ? 1 : 0);
-  // raise the dynamic-LDS limit once per kernel instance (never inside a
-  // stream capture that follows an eager launch of the same shape)
-  static size_t lds_limit[8] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024,
-                                64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
-  if (smem > lds_limit[v]) {
-    hipError_t e = hipFuncSetAttribute(fast_variant_fn<Real, N, RM>(split, pair, nt),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(smem));
+  // raise the dynamic-LDS limit once per kernel instance
+  static std::atomic<size_t> lds_limit[8] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024,
+                                             64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
+  {
+    const hipError_t e = raise_lds_limit(lds_limit[v], fast_variant_fn<Real, N, RM>(split, pair, nt), smem);
     if (e != hipSuccess) return e;
-    lds_limit[v] = smem;
   }
   const int64_t units = pair ? (static_cast<int64_t>(p.nlines) + 1) / 2 : p.nlines;
   const unsigned grid = static_cast<unsigned>((units + B - 1) / B);

@@ -28,7 +28,7 @@ struct PassSpec {
   int xcd = 0;           // XCD-contiguous block numbering
   int swz = 0;           // column-major walk of the block grid
   int split = 1;         // exchanges in real / imaginary halves (half the LDS; FP64, FP32 n >= 1024)
-  int dual = 0;          // FP32 C2C: two adjacent lines per lane (16-byte accesses)
+  int dual = 1;          // FP32 C2C: two adjacent lines per lane (16-byte accesses) where they apply
 };
 
 // device table exp(-2 pi i e / n), e in [0, n): evaluated in long double,

@@ -154,8 +154,9 @@ int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int d
  * of the block grid, radix-stage exchanges through LDS in real and
  * imaginary halves (default on; half the LDS per line; FP64 lines and FP32
  * lines of 1024 points or more), and "fwd.<i>.dual" = two adjacent FP32
- * lines per lane with 16-byte accesses (C2C lanes-across-lines passes whose
- * consecutive lines are adjacent row elements; elsewhere ignored) of pass i. */
+ * lines per lane with 16-byte accesses (default on; C2C lanes-across-lines
+ * passes whose consecutive lines are adjacent row elements; elsewhere
+ * ignored) of pass i. */
 int dfft_plan_set_option(dfft_plan* plan, const char* key, long value);
 int dfft_plan_get_option(const dfft_plan* plan, const char* key, long* value);
 /* Autotuner: times every launch shape of every pass and the chunk sizes on the
