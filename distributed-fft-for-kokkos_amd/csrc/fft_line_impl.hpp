@@ -1049,10 +1049,10 @@ const void* fast_variant_fn(bool split, bool pair, bool nt) {
 // Two lines per lane applies to FP32 C2C lanes-across-lines passes of
 // power-of-two lengths >= 128 whose consecutive lines are adjacent elements
 // of one row on both sides (the fastest batch axis is the innermost storage
-// axis, not split over ranks, with even row lengths and 16-byte aligned rank
-// buffers, so every line pair is one aligned 16-byte element pair), with an
-// even number of lines per block, of lines, of the first line and of the
-// unpadded row length.
+// axis, not split over ranks, with even row lengths (pitches) and 16-byte
+// aligned rank buffers, so every line pair is one aligned 16-byte element
+// pair), with an even number of lines per block, of lines and of the first
+// line.
 template <typename Real, int N> constexpr bool dual_kernel_exists() {
   return std::is_same<Real, float>::value && N >= 128 && (N & (N - 1)) == 0;
 }
@@ -1060,7 +1060,10 @@ template <typename Real, int N> constexpr bool dual_kernel_exists() {
 inline bool dual_lines_apply_impl(const LinePass& p, int B) {
   if (!p.dual_lines || p.kind != kC2C || !p.lines_fast || p.nbatch < 1 || B < 2 || B % 2) return false;
   const int j = p.nbatch - 1, fa = p.batch_axis[j];
-  if (p.batch_ext[j] % 2 || p.line_begin % 2 || p.nlines % 2 || (p.pad_real > 0 && p.pad_real % 2)) return false;
+  // an odd unpadded row length of a padded array is fine: the pair at the
+  // end of a row then covers one padding element, which lies inside the row
+  // pitch and is never read by any pass
+  if (p.batch_ext[j] % 2 || p.line_begin % 2 || p.nlines % 2) return false;
   for (const DDesc* d : {&p.in, &p.out}) {
     const int inner = d->layout_left ? d->map[0] : d->map[d->ndim - 1];
     if (inner != fa || d->ntopo[fa] != 1) return false;
