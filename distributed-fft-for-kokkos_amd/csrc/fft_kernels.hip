@@ -58,14 +58,17 @@ __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* 
         }
       }
     } else {
+      // complex64 contiguous lines move 16 bytes per lane through LDS
+      // (complex128 elements are 16 bytes already)
+      constexpr bool kStage = std::is_same<Real, float>::value;
       switch (p.n) {
-        case 16: fast_line_body<Real, 16, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-        case 32: fast_line_body<Real, 32, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-        case 64: fast_line_body<Real, 64, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-        case 96: fast_line_body<Real, 96, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-        case 128: fast_line_body<Real, 128, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-        case 160: fast_line_body<Real, 160, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
-        case 256: fast_line_body<Real, 256, 8, NT, false>(p, blk, p.lines_per_block, lds); break;
+        case 16: fast_line_body<Real, 16, 8, NT, false, kStage>(p, blk, p.lines_per_block, lds); break;
+        case 32: fast_line_body<Real, 32, 8, NT, false, kStage>(p, blk, p.lines_per_block, lds); break;
+        case 64: fast_line_body<Real, 64, 8, NT, false, kStage>(p, blk, p.lines_per_block, lds); break;
+        case 96: fast_line_body<Real, 96, 8, NT, false, kStage>(p, blk, p.lines_per_block, lds); break;
+        case 128: fast_line_body<Real, 128, 8, NT, false, kStage>(p, blk, p.lines_per_block, lds); break;
+        case 160: fast_line_body<Real, 160, 8, NT, false, kStage>(p, blk, p.lines_per_block, lds); break;
+        case 256: fast_line_body<Real, 256, 8, NT, false, kStage>(p, blk, p.lines_per_block, lds); break;
         default: break;
       }
     }

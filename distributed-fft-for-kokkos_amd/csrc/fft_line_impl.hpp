@@ -654,8 +654,9 @@ __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, 
 // One block's share of a line pass: lines [blk*B, blk*B + B) of p, T = N/R
 // lanes per line.  Lanes beyond B*T (a block shape shared by several line
 // lengths) run the stages on zeros in a spare LDS slot (index B) so every
-// lane reaches every barrier; they load and store nothing.
-template <typename Real, int N, int RM, bool NT, bool SPLIT>
+// lane reaches every barrier; they load and store nothing.  STAGE (complex64
+// grouped passes) moves contiguous lines 16 bytes per lane through LDS.
+template <typename Real, int N, int RM, bool NT, bool SPLIT, bool STAGE = false>
 __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
   using C = cplx_t<Real>;
   using Cfg = FastCfg<Real, N, RM>;
@@ -695,7 +696,40 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
   C v[R];
   constexpr int r0 = Cfg::P.rad[0];
   constexpr int NB0 = R / r0;
-  if (p.kind == kC2C) {
+  typedef Real V4 __attribute__((ext_vector_type(4)));
+  static_assert(!STAGE || (std::is_same<Real, float>::value && !SPLIT && R % 2 == 0),
+                "staged lines are complex64 with complex LDS exchanges");
+  bool vin = false;
+  if constexpr (STAGE) {
+    // contiguous C2C lines that start on a 16-byte boundary are read as
+    // element pairs (lane t takes pairs t, t + T, ...: T lanes cover 16*T
+    // adjacent bytes per access) and regrouped through the line's LDS buffer
+    C* lc = static_cast<C*>(buf);
+    vin = valid && p.kind == kC2C && io.is == 1 && (reinterpret_cast<uintptr_t>(io.ib) & 15) == 0;
+    if (vin) {
+      const V4* src = reinterpret_cast<const V4*>(io.ib) + t;
+#pragma unroll
+      for (int k = 0; k < R / 2; ++k) {
+        const V4 x = gload<NT>(src + k * T);
+        const int e = 2 * (t + k * T);
+        lc[Cfg::lidx(e)] = C{x.x, x.y};
+        lc[Cfg::lidx(e + 1)] = C{x.z, x.w};
+      }
+    }
+    if (p.kind == kC2C) __syncthreads();  // p is the same member for the whole block
+    if (vin) {
+#pragma unroll
+      for (int q = 0; q < NB0; ++q) {
+#pragma unroll
+        for (int j = 0; j < r0; ++j) {
+          const C x = lc[Cfg::lidx(t * NB0 + q + j * (N / r0))];
+          v[q * r0 + j] = p.inverse ? cconj(x) : x;
+        }
+      }
+    }
+  }
+  if (vin) {
+  } else if (p.kind == kC2C) {
     // C2C lines are walked with pointer increments (no 64-bit product per element)
 #pragma unroll
     for (int q = 0; q < NB0; ++q) {
@@ -717,9 +751,48 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
     }
   }
   StageLoop<Real, N, RM, SPLIT, 0>::run(v, t, tw, buf);
-  final_regroup<Real, N, RM, SPLIT>(v, t, buf);
+  bool vout = false;
+  if constexpr (STAGE) {
+    // the last stage's outputs go to LDS in natural order (this replaces
+    // final_regroup); contiguous aligned C2C outputs are stored as element
+    // pairs, any other line reads back positions t + j*T
+    constexpr int rl = Cfg::P.rad[Cfg::NST - 1];
+    constexpr int NBl = R / rl;
+    C* lc = static_cast<C*>(buf);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NBl; ++q) {
+#pragma unroll
+      for (int j = 0; j < rl; ++j) lc[Cfg::lidx(t * NBl + q + j * (N / rl))] = v[q * rl + j];
+    }
+    __syncthreads();
+    vout = valid && p.kind == kC2C && io.os == 1 && (reinterpret_cast<uintptr_t>(io.ob) & 15) == 0;
+    if (vout) {
+      V4* dst = reinterpret_cast<V4*>(io.ob) + t;
+#pragma unroll
+      for (int k = 0; k < R / 2; ++k) {
+        const int e = 2 * (t + k * T);
+        C a = lc[Cfg::lidx(e)], c = lc[Cfg::lidx(e + 1)];
+        if (p.inverse) {
+          a = cconj(a);
+          c = cconj(c);
+        }
+        V4 x;
+        x.x = a.x * io.scale;
+        x.y = a.y * io.scale;
+        x.z = c.x * io.scale;
+        x.w = c.y * io.scale;
+        gstore<NT>(dst + k * T, x);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = lc[Cfg::lidx(t + j * T)];
+    }
+  } else {
+    final_regroup<Real, N, RM, SPLIT>(v, t, buf);
+  }
 
-  if (valid) {
+  if (valid && !vout) {
     if (p.kind == kC2C) {
       C* dst = reinterpret_cast<C*>(io.ob) + static_cast<int64_t>(t) * io.os;
       const int64_t step = static_cast<int64_t>(T) * io.os;
