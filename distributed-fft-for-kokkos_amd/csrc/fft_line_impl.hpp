@@ -729,6 +729,7 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
     }
   }
   if (vin) {
+    // loaded above, through LDS
   } else if (p.kind == kC2C) {
     // C2C lines are walked with pointer increments (no 64-bit product per element)
 #pragma unroll
