@@ -129,6 +129,22 @@ hipError_t ns_gradients(const NsGrid& g, const NsBlock& b, const double2* y3, do
   return hipGetLastError();
 }
 
+// Streaming (nontemporal) element access of the spectral fields: the stage
+// update reads and writes 18-24 fields once each (2.4-3.2 GB at 256^3), far
+// beyond the caches, so its lines need not displace anything (256^3 step
+// 9.17 -> 8.89 ms; the 128^3 adjoint unchanged).
+typedef double StreamD2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 stream_load(const double2* p) {
+  const StreamD2 v = __builtin_nontemporal_load(reinterpret_cast<const StreamD2*>(p));
+  return double2{v.x, v.y};
+}
+__device__ __forceinline__ void stream_store(double2* p, double2 x) {
+  StreamD2 v;
+  v.x = x.x;
+  v.y = x.y;
+  __builtin_nontemporal_store(v, reinterpret_cast<StreamD2*>(p));
+}
+
 __global__ void __launch_bounds__(256) rhs_rk_kernel(const NsGrid g, const NsBlock b, const NsStage st) {
   const int64_t m_tot = static_cast<int64_t>(b.ext[0]) * b.ext[1] * b.ext[2];
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -137,21 +153,22 @@ __global__ void __launch_bounds__(256) rhs_rk_kernel(const NsGrid g, const NsBlo
     double2 nl[3];
     if (st.form == 0) {
       // N_i = i sum_j k_j P_ij with P = [uu uv uw; uv vv vw; uw vw ww]
-      const double2 p0 = st.nl[m], p1 = st.nl[m_tot + m], p2 = st.nl[2 * m_tot + m];
-      const double2 p3 = st.nl[3 * m_tot + m], p4 = st.nl[4 * m_tot + m], p5 = st.nl[5 * m_tot + m];
+      const double2 p0 = stream_load(st.nl + m), p1 = stream_load(st.nl + (m_tot + m));
+      const double2 p2 = stream_load(st.nl + (2 * m_tot + m)), p3 = stream_load(st.nl + (3 * m_tot + m));
+      const double2 p4 = stream_load(st.nl + (4 * m_tot + m)), p5 = stream_load(st.nl + (5 * m_tot + m));
       nl[0] = ci(cadd2(cadd2(cscale(p0, md.kx), cscale(p1, md.ky)), cscale(p2, md.kz)));
       nl[1] = ci(cadd2(cadd2(cscale(p1, md.kx), cscale(p3, md.ky)), cscale(p4, md.kz)));
       nl[2] = ci(cadd2(cadd2(cscale(p2, md.kx), cscale(p4, md.ky)), cscale(p5, md.kz)));
     } else {
-      nl[0] = st.nl[m];
-      nl[1] = st.nl[m_tot + m];
-      nl[2] = st.nl[2 * m_tot + m];
+      nl[0] = stream_load(st.nl + m);
+      nl[1] = stream_load(st.nl + (m_tot + m));
+      nl[2] = stream_load(st.nl + (2 * m_tot + m));
     }
     // dealias, combine (-adv - nu k^2 u), project (main.cpp:655-665, 780-810)
     double2 y[3], r[3];
     const double visc = -g.nu * md.ksq;
     for (int c = 0; c < 3; ++c) {
-      y[c] = cscale(st.y[c * m_tot + m], md.mask);
+      y[c] = cscale(stream_load(st.y + (c * m_tot + m)), md.mask);
       r[c] = double2{-md.mask * nl[c].x + visc * y[c].x, -md.mask * nl[c].y + visc * y[c].y};
     }
     project(md, r);
@@ -161,22 +178,22 @@ __global__ void __launch_bounds__(256) rhs_rk_kernel(const NsGrid g, const NsBlo
       const double2 kk = cscale(r[c], g.dt);
       double2 yn;
       if (st.stage == 0) {
-        st.yc[e] = y[c];
-        st.k1[e] = kk;
+        stream_store(st.yc + e, y[c]);
+        stream_store(st.k1 + e, kk);
         yn = cadd2(y[c], cscale(kk, 0.5));
       } else if (st.stage == 1) {
-        st.k2[e] = kk;
-        yn = cadd2(st.yc[e], cscale(kk, 0.5));
+        stream_store(st.k2 + e, kk);
+        yn = cadd2(stream_load(st.yc + e), cscale(kk, 0.5));
       } else if (st.stage == 2) {
-        st.k3[e] = kk;
-        yn = cadd2(st.yc[e], kk);
+        stream_store(st.k3 + e, kk);
+        yn = cadd2(stream_load(st.yc + e), kk);
       } else {
-        const double2 k1 = st.k1[e], k2 = st.k2[e], k3 = st.k3[e];
+        const double2 k1 = stream_load(st.k1 + e), k2 = stream_load(st.k2 + e), k3 = stream_load(st.k3 + e);
         const double2 sum = double2{(k1.x + 2.0 * k2.x + 2.0 * k3.x + kk.x) / 6.0,
                                     (k1.y + 2.0 * k2.y + 2.0 * k3.y + kk.y) / 6.0};
-        yn = cadd2(st.yc[e], sum);
+        yn = cadd2(stream_load(st.yc + e), sum);
       }
-      st.y[e] = cscale(yn, md.mask);
+      stream_store(st.y + e, cscale(yn, md.mask));
     }
   }
 }
