@@ -167,6 +167,7 @@ int dfft_plan_workspace_bytes(const dfft_plan* plan, size_t* bytes) {
 int dfft_execute(dfft_plan* plan, void* const* in, void* const* out, int direction, void* stream) {
   return guarded([&] {
     need(plan, "null plan");
+    plan->impl->check_buffers(in, out, direction);
     plan->impl->execute(in, out, direction, static_cast<hipStream_t>(stream));
   });
 }
@@ -174,6 +175,9 @@ int dfft_execute(dfft_plan* plan, void* const* in, void* const* out, int directi
 int dfft_execute_adjoint(dfft_plan* plan, void* const* in, void* const* out, int direction, void* stream) {
   return guarded([&] {
     need(plan, "null plan");
+    need(direction == DFFT_FORWARD || direction == DFFT_BACKWARD, "Invalid FFT direction specified.");
+    // the adjoint of `direction` reads and writes what the opposite transform does
+    plan->impl->check_buffers(in, out, direction == DFFT_FORWARD ? DFFT_BACKWARD : DFFT_FORWARD);
     plan->impl->execute_adjoint(in, out, direction, static_cast<hipStream_t>(stream));
   });
 }
@@ -189,6 +193,7 @@ int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int d
                        float* ms_per_pass, int capacity, int* npasses) {
   return guarded([&] {
     need(plan && npasses, "null pointer");
+    plan->impl->check_buffers(in, out, direction);
     auto ms = plan->impl->execute_timed(in, out, direction, static_cast<hipStream_t>(stream));
     *npasses = static_cast<int>(ms.size());
     for (std::size_t i = 0; i < ms.size() && static_cast<int>(i) < capacity; ++i) ms_per_pass[i] = ms[i];
@@ -467,6 +472,7 @@ int dfft_ns_init_taylor_green(dfft_ns* ns, double v0, void* stream) {
 int dfft_ns_set_physical(dfft_ns* ns, void* const* u3, void* stream) {
   return guarded([&] {
     need(ns && u3, "null pointer");
+    ns->impl->check_fields(u3, false, "physical velocity");
     ns->impl->set_physical(u3, static_cast<hipStream_t>(stream));
   });
 }
@@ -474,6 +480,7 @@ int dfft_ns_set_physical(dfft_ns* ns, void* const* u3, void* stream) {
 int dfft_ns_set_state(dfft_ns* ns, void* const* y3, void* stream) {
   return guarded([&] {
     need(ns && y3, "null pointer");
+    ns->impl->check_fields(y3, true, "state");
     ns->impl->set_state(y3, static_cast<hipStream_t>(stream));
   });
 }
@@ -481,6 +488,7 @@ int dfft_ns_set_state(dfft_ns* ns, void* const* y3, void* stream) {
 int dfft_ns_get_state(dfft_ns* ns, void* const* y3, void* stream) {
   return guarded([&] {
     need(ns && y3, "null pointer");
+    ns->impl->check_fields(y3, true, "state");
     ns->impl->get_state(y3, static_cast<hipStream_t>(stream));
   });
 }
@@ -509,6 +517,7 @@ int dfft_ns_energy(dfft_ns* ns, double* energy, void* stream) {
 int dfft_ns_velocity(dfft_ns* ns, void* const* u3, void* stream) {
   return guarded([&] {
     need(ns && u3, "null pointer");
+    ns->impl->check_fields(u3, false, "velocity");
     ns->impl->velocity(u3, static_cast<hipStream_t>(stream));
   });
 }
@@ -516,6 +525,7 @@ int dfft_ns_velocity(dfft_ns* ns, void* const* u3, void* stream) {
 int dfft_ns_vorticity(dfft_ns* ns, void* const* w3, void* stream) {
   return guarded([&] {
     need(ns && w3, "null pointer");
+    ns->impl->check_fields(w3, false, "vorticity");
     ns->impl->vorticity(w3, static_cast<hipStream_t>(stream));
   });
 }
@@ -538,6 +548,8 @@ int dfft_ns_diag(dfft_ns* ns, const char* out_dir, int diag_it, double* energy, 
 int dfft_ns_rhs(dfft_ns* ns, void* const* y3, void* const* out3, void* stream) {
   return guarded([&] {
     need(ns && y3 && out3, "null pointer");
+    ns->impl->check_fields(y3, true, "state");
+    ns->impl->check_fields(out3, true, "right-hand side");
     ns->impl->apply_rhs(y3, out3, static_cast<hipStream_t>(stream));
   });
 }
@@ -545,6 +557,9 @@ int dfft_ns_rhs(dfft_ns* ns, void* const* y3, void* const* out3, void* stream) {
 int dfft_ns_rhs_tangent(dfft_ns* ns, void* const* y3, void* const* d3, void* const* out3, void* stream) {
   return guarded([&] {
     need(ns && y3 && d3 && out3, "null pointer");
+    ns->impl->check_fields(y3, true, "state");
+    ns->impl->check_fields(d3, true, "direction");
+    ns->impl->check_fields(out3, true, "tangent");
     ns->impl->apply_rhs_tangent(y3, d3, out3, static_cast<hipStream_t>(stream));
   });
 }
@@ -552,6 +567,9 @@ int dfft_ns_rhs_tangent(dfft_ns* ns, void* const* y3, void* const* d3, void* con
 int dfft_ns_rhs_adjoint(dfft_ns* ns, void* const* y3, void* const* l3, void* const* out3, void* stream) {
   return guarded([&] {
     need(ns && y3 && l3 && out3, "null pointer");
+    ns->impl->check_fields(y3, true, "state");
+    ns->impl->check_fields(l3, true, "adjoint variable");
+    ns->impl->check_fields(out3, true, "adjoint");
     ns->impl->apply_rhs_adjoint(y3, l3, out3, static_cast<hipStream_t>(stream));
   });
 }
@@ -560,6 +578,8 @@ int dfft_ns_gradient(dfft_ns* ns, int nsteps, int checkpoint_every, void* const*
                      void* stream) {
   return guarded([&] {
     need(ns && u0 && grad && J, "null pointer");
+    ns->impl->check_fields(u0, false, "initial velocity");
+    ns->impl->check_fields(grad, false, "gradient");
     *J = ns->impl->gradient(nsteps, checkpoint_every, u0, grad, static_cast<hipStream_t>(stream));
   });
 }

@@ -220,7 +220,7 @@ hipError_t launch_generic(LinePass p, hipStream_t s) {
   int B = p.n >= 1024 ? 1 : 1024 / p.n;
   while (B > 1 && static_cast<size_t>(2) * B * p.n * sizeof(C) > 64 * 1024) B /= 2;
   const size_t smem = static_cast<size_t>(2) * B * p.n * sizeof(C);
-  if (smem > 160 * 1024) return hipErrorInvalidValue;
+  if (smem > kGenericLdsMax) return hipErrorInvalidValue;
   static std::atomic<size_t> lds_limit{64 * 1024};
   {
     const hipError_t e = raise_lds_limit(lds_limit, reinterpret_cast<const void*>(&fft_line_generic<Real>), smem);
@@ -352,6 +352,13 @@ bool has_fast_line_kernel(int n) {
     default:
       return false;
   }
+}
+
+bool line_length_supported(int n, bool fp64) {
+  if (n < 1) return false;
+  if (has_fast_line_kernel(n)) return true;
+  const size_t csize = fp64 ? sizeof(double2) : sizeof(float2);
+  return static_cast<size_t>(2) * static_cast<size_t>(n) * csize <= kGenericLdsMax;
 }
 
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s) {

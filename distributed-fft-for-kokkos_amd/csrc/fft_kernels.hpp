@@ -78,6 +78,7 @@ struct RemapArgs {
 // Grouped line pass (one launch for the same pass of many transforms, line
 // lengths 16, 32, 64, 96, 128, 160, 256 in any mix).
 constexpr int kGroupThreads = 256;
+constexpr size_t kGenericLdsMax = 160 * 1024;  // LDS of one gfx950 workgroup
 constexpr size_t kGroupLds = 64 * 1024;  // dynamic LDS of a grouped launch, at most
 // lines per block and dynamic LDS bytes of the grouped kernel for length n
 // with at most lds_cap bytes of LDS per block; false if n has no grouped kernel
@@ -94,6 +95,9 @@ int group_grid(size_t lds_bytes, bool fp64, bool dual);
 hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
                              int grid, size_t lds_bytes, bool fp64, bool nontemporal, bool dual, hipStream_t s);
 bool has_fast_line_kernel(int n);
+// true if a line of length n has a kernel: the fast set, or the generic
+// kernel (one line of n complex values, twice, in at most 160 KB of LDS)
+bool line_length_supported(int n, bool fp64);
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
 hipError_t launch_remap(const RemapArgs& a, int esize, hipStream_t s);
 

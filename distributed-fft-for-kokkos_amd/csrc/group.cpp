@@ -4,6 +4,7 @@
 #include <cmath>
 #include <string>
 
+#include "buffer_check.hpp"
 #include "plan.hpp"
 
 namespace dfft {
@@ -99,6 +100,11 @@ Group::Group(int count, const dfft_desc* descs) {
         case DFFT_NORM_FORWARD: m.scale_f = 1.0 / n; m.scale_b = 1.0; break;
         case DFFT_NORM_ORTHO: m.scale_f = m.scale_b = 1.0 / std::sqrt(n); break;
         default: break;
+      }
+      for (int a : axes) {
+        const int len = static_cast<int>(m.gin[a]);
+        if (!line_length_supported(len, fp64_))
+          throw Error(DFFT_ERROR_UNSUPPORTED, who + "no line kernel for transform length " + std::to_string(len));
       }
       for (int a : axes) {
         const int len = static_cast<int>(m.gin[a]);
@@ -427,6 +433,15 @@ void Group::execute(void* const* in, void* const* out, int direction, bool use_g
   if (cur != device_) check(hipSetDevice(device_), "hipSetDevice");
   DirState& ds = dir_[dir];
   if (!ds.built || ds.in != vin || ds.out != vout) {
+    // a new pointer set: every member's allocations must hold its arrays
+    const std::size_t cs = fp64_ ? 16 : 8;
+    for (std::size_t i = 0; i < members_.size(); ++i) {
+      const Member& m = members_[i];
+      const std::size_t nreal = volume(m.gin) * (m.real ? cs / 2 : cs), ncplx = volume(m.gout) * cs;
+      const std::string who = "group member " + std::to_string(i);
+      check_device_extent(vin[i], dir == 0 ? nreal : ncplx, who + " input");
+      check_device_extent(vout[i], dir == 0 ? ncplx : nreal, who + " output");
+    }
     if (ds.exec) {
       check(hipStreamSynchronize(graph_stream_), "hipStreamSynchronize");
       (void)hipGraphExecDestroy(ds.exec);

@@ -2,6 +2,8 @@
 // examples/navier-stokes-MPI-batched/main.cpp) over the fused plans.
 #include "navier_stokes.hpp"
 
+#include "buffer_check.hpp"
+
 #include <cmath>
 #include <cstring>
 
@@ -102,6 +104,14 @@ int64_t NavierStokes::spec_pts(int r) const {
 }
 int64_t NavierStokes::phys_pts(int r) const {
   return int64_t(pblk_[r].ext[0]) * pblk_[r].ext[1] * pblk_[r].ext[2];
+}
+
+void NavierStokes::check_fields(void* const* f3, bool spectral, const char* what) const {
+  if (f3 == nullptr) throw Error(DFFT_ERROR_INVALID_ARGUMENT, std::string(what) + ": null pointer table");
+  for (int r = 0; r < P_; ++r) {
+    const std::size_t bytes = 3 * static_cast<std::size_t>(spectral ? spec_pts(r) * 16 : phys_pts(r) * 8);
+    check_device_extent(f3[r], bytes, std::string(what) + " of rank " + std::to_string(r));
+  }
 }
 
 std::vector<void*> NavierStokes::alloc_set(bool spectral, int nf) {

@@ -39,6 +39,9 @@ class NavierStokes {
   const NsBlock& spectral_block(int r) const { return sblk_.at(r); }
   const NsBlock& physical_block(int r) const { return pblk_.at(r); }
   int nranks() const { return P_; }
+  // throws DFFT_ERROR_MISMATCH unless every rank's allocation holds 3 fields
+  // of its spectral (complex) or physical (real) block
+  void check_fields(void* const* f3, bool spectral, const char* what) const;
 
   // diagnostics (ns_diag.cpp; main.cpp:812-935)
   void velocity(void* const* u3, hipStream_t s);   // physical velocity of the state

@@ -12,6 +12,8 @@
 //    and one exchange remap per transpose block.
 #include "plan.hpp"
 
+#include "buffer_check.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -127,7 +129,37 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
       throw Error(DFFT_ERROR_UNSUPPORTED, "block-walk execution needs a slab or pencil plan");
   }
 
+  for (std::size_t k = 0; k < axes_.size(); ++k) {
+    const std::size_t n = fft_extents_[k];
+    if (!line_length_supported(static_cast<int>(n), fp64_))
+      throw Error(DFFT_ERROR_UNSUPPORTED, "no line kernel for transform length " + std::to_string(n) +
+                                              (fp64_ ? " in FP64" : " in FP32") +
+                                              " (lengths outside the fast set must fit one line in LDS)");
+  }
+
   check_hip(hipGetDevice(&device_), "hipGetDevice");
+  try {
+    allocate();
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void Plan::release() {
+  for (auto& kv : tw_) (void)hipFree(kv.second);
+  tw_.clear();
+  if (scratch_) (void)hipFree(scratch_);
+  scratch_ = nullptr;
+  for (auto& v : walk_bufs_) {
+    for (void* p : v) (void)hipFree(p);
+    v.clear();
+  }
+}
+
+// device memory of the plan: twiddle tables, then the fused executor's
+// scratch array or the block walk's ping-pong buffers
+void Plan::allocate() {
   // twiddle tables exp(-2 pi i e / n), evaluated in long double and rounded
   // once to FP64 -- for both precisions: the kernels form the powers of a
   // butterfly in FP64 and round each once to the compute precision
@@ -139,7 +171,7 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
   }
 
   const std::size_t cs = fp64_ ? 16 : 8;
-  if (d.exec_mode == DFFT_EXEC_FUSED) {
+  if (d_.exec_mode == DFFT_EXEC_FUSED) {
     plan_fused();
     // the spectrum (gout, complex) and, for an in-place backward call, the
     // input-shaped result (gin, real for R2C) both live in the scratch
@@ -166,12 +198,7 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
   }
 }
 
-Plan::~Plan() {
-  for (auto& kv : tw_) (void)hipFree(kv.second);
-  if (scratch_) (void)hipFree(scratch_);
-  for (auto& v : walk_bufs_)
-    for (void* p : v) (void)hipFree(p);
-}
+Plan::~Plan() { release(); }
 
 void Plan::local_starts(std::size_t r, int which, Vec& ext, Vec& st) const {
   if (r >= P_) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "rank must be less than topology size.");
@@ -503,6 +530,20 @@ void Plan::execute(void* const* in, void* const* out, int direction, hipStream_t
   else run_fused(in, out, forward, stream);
 }
 
+void Plan::check_buffers(void* const* in, void* const* out, int direction) const {
+  if (in == nullptr || out == nullptr) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null pointer table");
+  if (direction != DFFT_FORWARD && direction != DFFT_BACKWARD)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "Invalid FFT direction specified.");
+  const bool forward = direction == DFFT_FORWARD;
+  const std::size_t cs = fp64_ ? 16 : 8, rs = real_ ? cs / 2 : cs;
+  for (std::size_t r = 0; r < P_; ++r) {
+    const std::size_t nin = total_size(in_local_[r]) * rs, nout = total_size(out_local_[r]) * cs;
+    const std::string who = " of rank " + std::to_string(r);
+    check_device_extent(in[r], forward ? nin : nout, "input View" + who);
+    check_device_extent(out[r], forward ? nout : nin, "output View" + who);
+  }
+}
+
 // The adjoint of a transform is the opposite-direction pass sequence with
 // the transform's own scale: (s F)^* = s n F^{-1} for C2C.  For R2C the
 // half-spectrum side counts each stored entry once, so
@@ -633,32 +674,10 @@ float Plan::time_direction(void* const* in, void* const* out, bool forward, hipS
 // lines per block x lane order) is timed on the pass's own descriptors; then
 // the chunk size of the first two passes.  Shapes the launcher rejects
 // (too many threads or too much LDS) are skipped before launch.
-// The allocation holding p must extend at least `bytes` past p (checked when
-// the runtime knows the allocation; memory it does not track is let through).
-static void check_extent(const void* p, std::size_t bytes, const char* what) {
-  if (p == nullptr || bytes == 0) return;
-  hipDeviceptr_t base = nullptr;
-  size_t size = 0;
-  if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess) {
-    (void)hipGetLastError();
-    return;
-  }
-  const char* b = static_cast<const char*>(base);
-  const char* q = static_cast<const char*>(p);
-  if (q + bytes > b + size)
-    throw Error(DFFT_ERROR_INVALID_ARGUMENT, std::string(what) + ": buffer smaller than the local array it must hold");
-}
-
 void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
   if (d_.exec_mode != DFFT_EXEC_FUSED) return;
-  {
-    // the tuner writes both sides: refuse arrays that cannot hold them
-    const std::size_t cs = fp64_ ? 16 : 8, rs = real_ ? cs / 2 : cs;
-    for (std::size_t r = 0; r < P_; ++r) {
-      check_extent(in[r], total_size(in_local_[r]) * rs, "autotune input");
-      check_extent(out[r], total_size(out_local_[r]) * cs, "autotune output");
-    }
-  }
+  // the tuner writes both sides: refuse arrays that cannot hold them
+  check_buffers(in, out, DFFT_FORWARD);
   check_hip(hipSetDevice(device_), "hipSetDevice");
   hipEvent_t e0, e1;
   check_hip(hipEventCreate(&e0), "hipEventCreate");

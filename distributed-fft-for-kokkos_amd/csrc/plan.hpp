@@ -43,6 +43,10 @@ class Plan {
   Plan& operator=(const Plan&) = delete;
 
   void execute(void* const* in, void* const* out, int direction, hipStream_t stream);
+  // throws DFFT_ERROR_MISMATCH unless every rank's in/out allocation can hold
+  // the local array that execute(direction) reads/writes (checked at the C
+  // boundary; internal callers pass the plan's own buffers)
+  void check_buffers(void* const* in, void* const* out, int direction) const;
   // adjoint of execute(direction) under the real inner product Re sum conj(a) b
   // of each side: in has the shape of that direction's output, out of its input
   void execute_adjoint(void* const* in, void* const* out, int direction, hipStream_t stream);
@@ -72,6 +76,8 @@ class Plan {
   std::size_t workspace_bytes() const { return workspace_bytes_; }
 
  private:
+  void allocate();
+  void release();
   DDesc user_desc(const Vec& ext, const Vec& topo, Layout tl, void* const* ptrs) const;
   // the plan scratch holding an array of extents `ext` with elements of
   // `elem_bytes` bytes; LayoutRight rows are padded to 128-byte multiples

@@ -397,7 +397,12 @@ class Plan:
     """``KokkosFFT::Distributed::Plan`` for P virtual ranks on one GPU."""
 
     def __init__(self, in_views: Sequence, out_views: Sequence, axes, in_topology, out_topology,
-                 norm="backward", exec_mode=EXEC_FUSED, global_in_extents=None, global_out_extents=None):
+                 norm="backward", exec_mode=EXEC_FUSED, global_in_extents=None, global_out_extents=None,
+                 layout=LayoutRight):
+        """``layout``: LayoutRight (row-major local arrays, the default) or
+        LayoutLeft (column-major: a view of extents (e0, e1, e2) is a tensor
+        with strides (1, e0, e0*e1), e.g. ``torch.empty(e[::-1]).permute(2, 1, 0)``)
+        -- the reference's view layout template parameter."""
         import torch
         lib = load_library()
         self._lib = lib
@@ -412,6 +417,11 @@ class Plan:
             raise DFFTError(ERROR_INVALID_ARGUMENT, "Plan: the output type must be complex")
         real = not in_dtype.is_complex
         fp64 = out_dtype == torch.complex128
+        want_in = (torch.float64 if fp64 else torch.float32) if real else out_dtype
+        if in_dtype != want_in:
+            raise DFFTError(ERROR_MISMATCH, f"Plan: input type {in_dtype} does not match output type {out_dtype}")
+        self.in_dtype, self.out_dtype = in_dtype, out_dtype
+        self.device = _device_of(list(in_views) + list(out_views))
         self.real, self.fp64 = real, fp64
         rank = in_views[0].dim()
         self.rank = rank
@@ -419,7 +429,8 @@ class Plan:
         gout = global_out_extents or _global_extents([list(v.shape) for v in out_views], tout)
         self.global_in_extents, self.global_out_extents = list(gin), list(gout)
         self.axes = [int(a) for a in axes]
-        d = _make_desc(gin, gout, self.axes, tin, tout, real, fp64, norm, exec_mode)
+        self.layout = layout.value if hasattr(layout, "value") else int(layout)
+        d = _make_desc(gin, gout, self.axes, tin, tout, real, fp64, norm, exec_mode, self.layout)
         _check(lib.dfft_plan_create(ctypes.byref(d), ctypes.byref(self._plan)))
         self.in_extents = [list(v.shape) for v in in_views]
         self.out_extents = [list(v.shape) for v in out_views]
@@ -467,8 +478,38 @@ class Plan:
         _check(self._lib.dfft_execute(self._plan, _ptrs(ins), _ptrs(outs), int(direction),
                                       _stream_handle(stream)))
 
+    def check_views(self, ins, outs, direction):
+        """The reference's ``good()`` (InternalPlan.hpp:109-142) for the arrays
+        that ``execute(direction)`` reads (``ins``) and writes (``outs``):
+        one per rank, with the plan's extents, element types, device and
+        contiguity.  Raises DFFTError(ERROR_MISMATCH) otherwise."""
+        d = Direction(direction)
+        fwd = d == Direction.forward
+        shapes_in, shapes_out = (self.in_extents, self.out_extents) if fwd else (self.out_extents, self.in_extents)
+        dt_in, dt_out = (self.in_dtype, self.out_dtype) if fwd else (self.out_dtype, self.in_dtype)
+        if len(ins) != self.nranks or len(outs) != self.nranks:
+            raise DFFTError(ERROR_MISMATCH, f"expected {self.nranks} input and output views, "
+                            f"got {len(ins)} and {len(outs)}")
+        for r in range(self.nranks):
+            for v, shape, dt, side in ((ins[r], shapes_in[r], dt_in, "input"), (outs[r], shapes_out[r], dt_out, "output")):
+                if list(v.shape) != list(shape):
+                    raise DFFTError(ERROR_MISMATCH, f"extents of {side} View for plan and execution are not "
+                                    f"identical: View {list(v.shape)}, Plan {list(shape)}")
+                if v.dtype != dt:
+                    raise DFFTError(ERROR_MISMATCH, f"{side} View of rank {r} has type {v.dtype}, the plan "
+                                    f"needs {dt}")
+                if v.numel():
+                    if not v.is_cuda or v.device.index != self.device:
+                        raise DFFTError(ERROR_MISMATCH, f"{side} View of rank {r} is on {v.device}, the plan "
+                                        f"runs on cuda:{self.device}")
+                    if not _is_layout_contiguous(v, self.layout):
+                        raise DFFTError(ERROR_INVALID_ARGUMENT, f"views must be contiguous in the plan's layout "
+                                        f"({'LayoutLeft' if self.layout == LAYOUT_LEFT else 'LayoutRight'})")
+
     def execute_adjoint(self, ins, outs, direction, stream=None):
         """Adjoint of ``execute(direction)``: ``ins`` are shaped like that direction's outputs."""
+        d = Direction(direction)
+        self.check_views(ins, outs, Direction.backward if d == Direction.forward else Direction.forward)
         _check(self._lib.dfft_execute_adjoint(self._plan, _ptrs(ins), _ptrs(outs), int(direction),
                                               _stream_handle(stream)))
 
@@ -479,6 +520,7 @@ class Plan:
 
     def execute_timed(self, ins, outs, direction, stream=None):
         """Execute with HIP events around every launch; returns ms per launch."""
+        self.check_views(ins, outs, direction)
         cap = 64
         buf = (ctypes.c_float * cap)()
         n = ctypes.c_int()
@@ -505,6 +547,7 @@ class Plan:
 
     def autotune(self, in_views, out_views, stream=None):
         """Measure launch shapes on these arrays (their contents are overwritten)."""
+        self.check_views(in_views, out_views, Direction.forward)
         _check(self._lib.dfft_plan_autotune(self._plan, _ptrs(in_views), _ptrs(out_views),
                                             _stream_handle(stream)))
 
@@ -539,17 +582,44 @@ class Group:
         elif not isinstance(axes[0], (list, tuple)):
             axes = [list(axes)] * n
         descs = (_Desc * n)()
+        self.in_shapes = [list(t.shape) for t in ins]
+        self.out_shapes = [list(t.shape) for t in outs]
+        self.in_dtypes = [t.dtype for t in ins]
+        self.out_dtypes = [t.dtype for t in outs]
+        self.device = _device_of(list(ins) + list(outs))
         for i, (a, b) in enumerate(zip(ins, outs)):
             if not b.dtype.is_complex:
                 raise DFFTError(ERROR_INVALID_ARGUMENT, "Group: the output type must be complex")
             real = not a.dtype.is_complex
             fp64 = b.dtype == torch.complex128
+            want_in = (torch.float64 if fp64 else torch.float32) if real else b.dtype
+            if a.dtype != want_in:
+                raise DFFTError(ERROR_MISMATCH, f"Group member {i}: input type {a.dtype} does not match "
+                                f"output type {b.dtype}")
             ones = [1] * a.dim()
             descs[i] = _make_desc(list(a.shape), list(b.shape), axes[i], ones, ones, real, fp64, norm, EXEC_FUSED)
         _check(lib.dfft_group_create(n, descs, ctypes.byref(self._h)))
         self.count = n
 
+    def check_views(self, ins, outs, direction):
+        """Member arrays of ``execute(direction)``: count, extents, types,
+        device and contiguity as at creation (raises ERROR_MISMATCH)."""
+        fwd = Direction(direction) == Direction.forward
+        if len(ins) != self.count or len(outs) != self.count:
+            raise DFFTError(ERROR_MISMATCH, f"expected {self.count} member arrays per side")
+        for i in range(self.count):
+            si, so = (self.in_shapes[i], self.out_shapes[i]) if fwd else (self.out_shapes[i], self.in_shapes[i])
+            di, do = (self.in_dtypes[i], self.out_dtypes[i]) if fwd else (self.out_dtypes[i], self.in_dtypes[i])
+            for v, shape, dt, side in ((ins[i], si, di, "input"), (outs[i], so, do, "output")):
+                if list(v.shape) != shape or v.dtype != dt:
+                    raise DFFTError(ERROR_MISMATCH, f"Group member {i} {side}: {list(v.shape)} {v.dtype}, "
+                                    f"the group needs {shape} {dt}")
+                if v.numel() and (not v.is_cuda or v.device.index != self.device or not v.is_contiguous()):
+                    raise DFFTError(ERROR_MISMATCH, f"Group member {i} {side} must be a contiguous tensor on "
+                                    f"cuda:{self.device}")
+
     def execute(self, ins, outs, direction, use_graph=False, stream=None):
+        self.check_views(ins, outs, direction)
         _check(self._lib.dfft_group_execute(self._h, _ptrs(ins), _ptrs(outs), int(direction), int(bool(use_graph)),
                                             _stream_handle(stream)))
 
@@ -568,6 +638,7 @@ class Group:
 
     def autotune(self, ins, outs, stream=None):
         """Time LDS caps and grid sizes on these arrays (their contents are overwritten)."""
+        self.check_views(ins, outs, Direction.forward)
         _check(self._lib.dfft_group_autotune(self._h, _ptrs(ins), _ptrs(outs), _stream_handle(stream)))
 
     def destroy(self):
@@ -580,6 +651,36 @@ class Group:
             self.destroy()
         except Exception:
             pass
+
+
+def _device_of(tensors):
+    """CUDA device index shared by the non-empty tensors (raises otherwise)."""
+    dev = None
+    for t in tensors:
+        if t is None or t.numel() == 0:
+            continue
+        if not t.is_cuda:
+            raise DFFTError(ERROR_MISMATCH, f"views must be device tensors, got one on {t.device}")
+        if dev is None:
+            dev = t.device.index
+        elif t.device.index != dev:
+            raise DFFTError(ERROR_MISMATCH, f"views on cuda:{dev} and {t.device}")
+    if dev is None:
+        import torch
+        dev = torch.cuda.current_device()
+    return dev
+
+
+def _is_layout_contiguous(t, layout):
+    """True if ``t`` is dense in LayoutRight (C) or LayoutLeft (Fortran) order."""
+    if layout == LAYOUT_RIGHT:
+        return t.is_contiguous()
+    expect = 1
+    for size, stride in zip(t.shape, t.stride()):
+        if size != 1 and stride != expect:
+            return False
+        expect *= size
+    return True
 
 
 def _global_extents(local_shapes, topology):
@@ -615,25 +716,17 @@ def _coord_to_rank(coord, t):
 
 def execute(plan: Plan, in_views, out_views, direction, stream=None):
     """``execute(plan, in, out, direction)`` (Plan.hpp:212-238)."""
-    d = Direction(direction)
+    try:
+        d = Direction(direction)
+    except ValueError:
+        raise DFFTError(ERROR_INVALID_ARGUMENT, "Invalid FFT direction specified.") from None
     if d == Direction.forward:
         if not out_views[0].dtype.is_complex:
             raise DFFTError(ERROR_MISMATCH, "Forward FFT operation requires complex output type.")
-        shapes_in, shapes_out = plan.in_extents, plan.out_extents
     else:
         if not in_views[0].dtype.is_complex:
             raise DFFTError(ERROR_MISMATCH, "Backward FFT operation requires complex input type.")
-        shapes_in, shapes_out = plan.out_extents, plan.in_extents
-    for r in range(plan.nranks):
-        if list(in_views[r].shape) != shapes_in[r]:
-            raise DFFTError(ERROR_MISMATCH, "extents of input View for plan and execution are not identical: "
-                            f"View {list(in_views[r].shape)}, Plan {shapes_in[r]}")
-        if list(out_views[r].shape) != shapes_out[r]:
-            raise DFFTError(ERROR_MISMATCH, "extents of output View for plan and execution are not identical: "
-                            f"View {list(out_views[r].shape)}, Plan {shapes_out[r]}")
-        for v in (in_views[r], out_views[r]):
-            if v.numel() and not v.is_contiguous():
-                raise DFFTError(ERROR_INVALID_ARGUMENT, "views must be contiguous (LayoutRight)")
+    plan.check_views(in_views, out_views, d)
     plan._execute(in_views, out_views, d, stream)
 
 
