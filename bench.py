@@ -7,10 +7,14 @@ complex64 is timed beside it.  Inputs: U[-1,1] real and imaginary parts,
 seeded, resident in HBM before the timed region.
 
 Prints one JSON line.  `value` is the FP64 round-trip time in ms (lower is
-better).  `roofline` is the dominant kernel's algorithmic bytes (input +
-output array of the pass) over its average duration, timed with HIP events
-on the launch stream; `cpu_baseline` times the CPU restatement of the plan
-(oracle, single thread for the FFT lines) on a bounded sample and scales it.
+better).  `roofline` is the line-pass kernel's algorithmic bytes per launch
+(the compulsory bytes of a transform, input + output array, shared by its
+three axis-pass launches) over the launches' average duration, timed with
+HIP events on the launch stream: frac = 2B / t over the passes' own time;
+`pass_efficiency` beside it counts each launch's own read + write of the
+array.  `cpu_baseline` times the oracle's OpenMP CPU executor of the
+reference's block sequence on the same 512^3 round trip (median of 10 after
+3 warm-ups, every host core).
 
 Further sections of the same line:
 * r2c_1024: 1024^3 R2C forward + C2R backward, slab (4,1,1) -> (1,4,1),
@@ -91,46 +95,76 @@ def pass_times(dfft, plan, ins, outs, backs, reps):
     return avg(fw), avg(bw)
 
 
-def cpu_baseline():
-    """Oracle (CPU restatement of the slab plan, P=4 virtual ranks) on a
-    bounded sample: one 256^3 complex128 forward+backward, scaled to 512^3 by
-    the N log N operation count."""
+def _median(xs):
+    xs = sorted(xs)
+    k = len(xs) // 2
+    return xs[k] if len(xs) % 2 else 0.5 * (xs[k - 1] + xs[k])
+
+
+def cpu_baseline(warmup=3, runs=10):
+    """The oracle's OpenMP CPU executor of the reference's block sequence
+    (oracle/cpu_plan.c: FFT2 along (y, z) per x-slab, pack -> all-to-all ->
+    unpack into the (y, z, x) mapped layout, FFT along x, local transpose back,
+    normalisation sweep) on the same 512^3 complex128 slab (4,1,1) -> (1,4,1)
+    round trip, P = 4 virtual ranks on every host core (cores / P threads per
+    rank).  Timed as distributed/perf_test/PerfTest_FFT3D.cpp:50-60 times the
+    reference: the call returns after every rank is done (fence), wall clock,
+    `warmup` untimed round trips, then the median of `runs`."""
     import numpy as np
-    from oracle import plan_exec as PE
+    from oracle import cpu_plan as CP
 
-    n = 256
-    g = [n] * 3
+    cores = CP.cores_available()
+    CP.set_threads(cores)
+    g = [N] * 3
+    plan = CP.CpuPlan(g, [0, 1, 2], [P, 1, 1], [1, P, 1])
     rng = np.random.default_rng(3)
-    x = rng.uniform(-1, 1, g) + 1j * rng.uniform(-1, 1, g)
-    plan = PE.OraclePlan(g, g, [0, 1, 2], [P, 1, 1], [1, P, 1])
-    ins = PE.scatter(x, g, [P, 1, 1])
-    t0 = time.perf_counter()
-    outs = plan.forward(ins)
-    plan.backward(outs)
-    dt = time.perf_counter() - t0
-    scale = (N ** 3 * math.log2(N ** 3)) / (n ** 3 * math.log2(n ** 3))
-    return {"value": dt * 1e3 * scale, "unit": "ms per 512^3 complex128 fwd+bwd (extrapolated)",
-            "cores": 1, "kind": "port",
-            "sample": f"oracle slab plan P=4, one {n}^3 complex128 fwd+bwd in {dt:.2f} s, "
-                      f"scaled x{scale:.1f} by N log N"}
+    ins = []
+    for e in plan.meta.in_local:
+        a = np.empty(e, np.complex128)
+        a.real = rng.uniform(-1, 1, e)
+        a.imag = rng.uniform(-1, 1, e)
+        ins.append(a)
+    outs = [np.empty(e, np.complex128) for e in plan.meta.out_local]
+    backs = [np.empty_like(a) for a in ins]
+    times = []
+    for i in range(warmup + runs):
+        t0 = time.perf_counter()
+        plan.forward(ins, outs)
+        plan.backward(outs, backs)
+        dt = time.perf_counter() - t0
+        if i >= warmup:
+            times.append(dt * 1e3)
+    err = max(float(np.abs(a - b).max()) for a, b in zip(ins, backs))
+    return {"value": round(_median(times), 2), "unit": "ms per 512^3 complex128 fwd+bwd round trip",
+            "cores": cores, "kind": "port", "threads_per_rank": cores / P, "runs": runs, "warmup": warmup,
+            "min_ms": round(min(times), 2), "max_ms": round(max(times), 2), "roundtrip_max_abs_err": err,
+            "sample": f"the full 512^3 complex128 slab P={P} workload: OpenMP CPU block walk of the oracle "
+                      f"(own radix-4 FFT, reference's pack/all2all/unpack and transposes), {cores} threads, "
+                      f"median of {runs} after {warmup} warm-ups"}
 
 
-def cpu_library_baseline():
-    """scipy.fft (pocketfft) on all host cores: a library CPU bound beside the
-    oracle (one 256^3 complex128 fftn + ifftn, scaled to 512^3 by N log N)."""
+def cpu_library_baseline(warmup=3, runs=10):
+    """scipy.fft (pocketfft) fftn + ifftn of one global 512^3 complex128 array
+    on all host cores: a library CPU bound beside the block walk (no
+    distribution, no exchanges)."""
     import numpy as np
     import scipy.fft as sf
-    n = 256
     cores = len(os.sched_getaffinity(0))
     rng = np.random.default_rng(3)
-    x = rng.uniform(-1, 1, (n,) * 3) + 1j * rng.uniform(-1, 1, (n,) * 3)
-    sf.ifftn(sf.fftn(x, workers=cores), workers=cores)  # warm-up
-    t0 = time.perf_counter()
-    sf.ifftn(sf.fftn(x, workers=cores), workers=cores)
-    dt = time.perf_counter() - t0
-    scale = (N ** 3 * math.log2(N ** 3)) / (n ** 3 * math.log2(n ** 3))
-    return {"value": dt * 1e3 * scale, "unit": "ms per 512^3 complex128 fwd+bwd (extrapolated)", "cores": cores,
-            "kind": "library", "sample": f"scipy.fft fftn+ifftn {n}^3 complex128 in {dt:.3f} s, x{scale:.1f}"}
+    x = np.empty((N,) * 3, np.complex128)
+    x.real = rng.uniform(-1, 1, x.shape)
+    x.imag = rng.uniform(-1, 1, x.shape)
+    times = []
+    for i in range(warmup + runs):
+        t0 = time.perf_counter()
+        sf.ifftn(sf.fftn(x, workers=cores), workers=cores)
+        dt = time.perf_counter() - t0
+        if i >= warmup:
+            times.append(dt * 1e3)
+    return {"value": round(_median(times), 2), "unit": "ms per 512^3 complex128 fwd+bwd", "cores": cores,
+            "kind": "library", "runs": runs, "warmup": warmup,
+            "sample": f"scipy.fft fftn+ifftn of one 512^3 complex128 array, workers={cores}, "
+                      f"median of {runs} after {warmup} warm-ups"}
 
 
 def rocfft_comparator(torch, dev, steps):
@@ -284,7 +318,8 @@ def bench_r2c_1024(torch, dfft, dev, steps, warmup, tune):
         achieved = sum(pass_bytes) / (sum(allp) * 1e-3) / 1e9
         res[tag] = {"ms": round(ms, 3), "fwd_pass_ms": [round(x, 4) for x in fw],
                     "bwd_pass_ms": [round(x, 4) for x in bw],
-                    "avg_pass_GBs": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "avg_pass_GBs": round(achieved, 1),
+                    "pass_efficiency": round(achieved / HBM_PEAK_GBS, 4),
                     "roofline_2B": round(2 * (real_b + half_b) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "roundtrip_rel_l2": err}
         log(f"[bench] r2c {tag}: {ms:.3f} ms, passes {fw} {bw}, {achieved:.0f} GB/s per pass")
@@ -430,16 +465,25 @@ def main():
     B = 2 * N ** 3 * esz  # input + output array of one transform (bytes)
     r64 = results["fp64"]
     allp = r64["fwd_pass_ms"] + r64["bwd_pass_ms"]
-    dom = max(allp)
-    pass_bytes = 2 * N ** 3 * esz  # each pass reads and writes the whole array
-    achieved = pass_bytes / (sum(allp) / len(allp)) / 1e6  # GB/s, average pass
+    avg_launch = sum(allp) / len(allp)
+    launches = len(r64["fwd_pass_ms"])  # line-pass launches per transform (one per axis)
+    # algorithmic bytes per launch: the transform's compulsory bytes B (input +
+    # output array) shared by its launches, so achieved / peak is the
+    # round-trip figure 2B / t over the passes' own time
+    alg_per_launch = B / launches
+    achieved = alg_per_launch / avg_launch / 1e6  # GB/s
+    pass_bytes = 2 * N ** 3 * esz  # each launch reads and writes the whole array
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 # HBM bytes per launch: FETCH_SIZE x2 + WRITE_SIZE x1 (copy-calibrated), rocprofv3 --pmc
                 "traffic": TRAFFIC_BYTES_PER_LAUNCH, "traffic_source": TRAFFIC_SOURCE,
-                "algorithmic_bytes_per_launch": pass_bytes,
-                "kernel": "fft_line_fast<double,512,R,NT> (the six axis passes of the round trip averaged)",
-                "dominant_pass_ms": round(dom, 4)}
+                "algorithmic_bytes_per_launch": alg_per_launch,
+                "algorithmic_bytes_per_transform": B, "launches_per_transform": launches,
+                "kernel": "fft_line_fast<double,512,R,NT> (the six axis-pass launches of the round trip)",
+                "avg_launch_ms": round(avg_launch, 4), "slowest_launch_ms": round(max(allp), 4),
+                # each launch's own read + write of the whole array over its duration
+                "pass_efficiency": round(pass_bytes / avg_launch / 1e6 / HBM_PEAK_GBS, 4),
+                "sweep_cap": round(1.0 / launches, 4)}
     line = {
         "metric": "3D FFT fwd+bwd ms & %HBM roofline @512^3 FP64 vs FP32; NS adjoint steps/s",
         "value": round(r64["ms"], 4),

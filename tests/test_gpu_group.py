@@ -117,8 +117,16 @@ def test_group_rejects_bad_members(dev):
     r = torch.zeros(8, 8, 8, dtype=torch.float32, device=dev)
     c = torch.zeros(8, 8, 5, dtype=torch.complex64, device=dev)
     g = dfft.Group([r], [c])
+    # arrays of the wrong type are refused before any pointer reaches the library
     with pytest.raises(dfft.DFFTError) as e:
         g.execute([c], [c], dfft.Direction.forward)
+    assert e.value.code == dfft.ERROR_MISMATCH
+    # an R2C member whose real input and half spectrum share their memory
+    base = torch.zeros(8 * 8 * 10, dtype=torch.float32, device=dev)
+    r_alias = base[:8 * 8 * 8].view(8, 8, 8)
+    c_alias = base.view(torch.complex64).view(8, 8, 5)
+    with pytest.raises(dfft.DFFTError) as e:
+        g.execute([r_alias], [c_alias], dfft.Direction.forward)
     assert e.value.code == dfft.ERROR_ALIAS
     with pytest.raises(dfft.DFFTError) as e:
         dfft.Group([r], [torch.zeros(8, 8, 6, dtype=torch.complex64, device=dev)])

@@ -274,3 +274,99 @@ def test_energy_spectrum_sums_to_energy(dev):
     w_hat = [1j * (g.KY * ys[2] - g.KZ * ys[1]), 1j * (g.KZ * ys[0] - g.KX * ys[2]),
              1j * (g.KX * ys[1] - g.KY * ys[0])]
     assert _rel(_gather(s, s.vorticity(), 0), np.stack([NS.backward(c, n) for c in w_hat])) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Taylor-Green 256^3 float64 on a 2 x 2 grid of virtual ranks (Re = 1600,
+# dt = 1e-3, divergence form, autotuned plans, hipGraph replay): the
+# production-size run of examples/navier-stokes-MPI/main.cpp:349-353, 572-666.
+#
+# The oracle cannot step 256^3 in test time, but the Taylor-Green state is
+# band-limited: the four stages of one RK4 step double the largest wavenumber
+# component each (1 -> 2 -> 4 -> 8 -> 16), and every product stays below
+# n/2 for n = 64 (no aliasing onto kept modes) and inside both dealias
+# cutoffs (21 at n = 64, 85 at n = 256).  One step at n = 64 therefore holds
+# the same coefficients as one step at n = 256, up to the factor (256/64)^3
+# of the unscaled forward transform, and every other coefficient is zero.
+# ---------------------------------------------------------------------------
+
+def _embed(small, n):
+    """Spectral field set of a small grid, scaled to the unscaled transform of
+    grid n and placed at the same wavenumbers of the (n, n, n/2+1) half spectrum."""
+    m = small.shape[1]
+    big = np.zeros(small.shape[:1] + (n, n, n // 2 + 1), dtype=np.complex128)
+    k = np.arange(-(m // 2) + 1, m // 2)  # the small grid's Nyquist mode stays out
+    kz = np.arange(0, m // 2)
+    ix, iz = k % m, kz
+    jx, jz = k % n, kz
+    scale = (n / m) ** 3
+    big[np.ix_(np.arange(small.shape[0]), jx, jx, jz)] = small[np.ix_(np.arange(small.shape[0]), ix, ix, iz)] * scale
+    return big
+
+
+def _max_divergence(n, y):
+    g = NS.Grid(n)
+    div = g.KX * y[0] + g.KY * y[1] + g.KZ * y[2]
+    return float(np.abs(div).max() / np.abs(y).max())
+
+
+def test_taylor_green_256_cubed_2x2_production_run(dev):
+    import dfft_ns
+    n, nu, dt = 256, 1.0 / 1600.0, 1e-3
+    s = _solver(n, 2, 2, dfft_ns.DIVERGENCE, nu, dt)
+    s.autotune()
+    s.init_taylor_green(1.0)
+    # E(0) = pi^3 and the initial state
+    assert abs(s.energy() - np.pi ** 3) < 1e-12 * np.pi ** 3
+    g16 = NS.Grid(16)
+    y16 = np.stack(NS.initial_spectrum(g16, *NS.taylor_green(g16)))
+    y0 = _gather(s, s.get_state(), 1)
+    assert _rel(y0, _embed(y16, n)) < 1e-12
+    # closed-form right-hand side at t = 0 (every mode has |k_i| <= 2)
+    d = _gather(s, s.rhs(s.get_state()), 1)
+    X, Y, Z = np.meshgrid(g16.x, g16.x, g16.x, indexing="ij")
+    u, v, _ = NS.taylor_green(g16)
+    want = [-np.sin(2 * X) * np.cos(2 * Z) / 8.0 - 3 * nu * u,
+            -np.sin(2 * Y) * np.cos(2 * Z) / 8.0 - 3 * nu * v,
+            (np.cos(2 * X) + np.cos(2 * Y)) * np.sin(2 * Z) / 8.0]
+    assert _rel(d, _embed(np.stack([NS.forward(c) for c in want]), n)) < 1e-12
+    del d
+    # one eager RK4 step against the oracle's step at n = 64
+    g64 = NS.Grid(64)
+    y64 = NS.rk4_step(g64, nu, dt, list(NS.initial_spectrum(g64, *NS.taylor_green(g64))))
+    s.step(1, use_graph=False)
+    y1 = _gather(s, s.get_state(), 1)
+    assert _rel(y1, _embed(np.stack(y64), n)) < 1e-12
+    assert abs(s.energy() - NS.kinetic_energy(g64, *y64)) < 1e-12 * np.pi ** 3
+    # two graph-replayed steps equal two eager steps bitwise and stay solenoidal
+    s.init_taylor_green(1.0)
+    s.step(2, use_graph=False)
+    eager = s.get_state()
+    e_eager = s.energy()
+    s.init_taylor_green(1.0)
+    s.step(1, use_graph=True)  # eager first step, then capture
+    s.step(1, use_graph=True)  # replay
+    replay = s.get_state()
+    for a, b in zip(eager, replay):
+        assert bool((a == b).all())
+    assert s.energy() == e_eager
+    y2 = _gather(s, replay, 1)
+    assert _max_divergence(n, y2) < 1e-12
+    # viscous decay: dE/dt(0) = -6 nu E(0) for Taylor-Green; over two steps the
+    # O(dt^2) remainder is far below the tolerance
+    rate = (e_eager - np.pi ** 3) / (2 * dt)
+    assert abs(rate + 6 * nu * np.pi ** 3) < 1e-3 * 6 * nu * np.pi ** 3
+    s.destroy()
+
+
+def test_taylor_green_256_cubed_2x2_inviscid_conserves_energy(dev):
+    """nu = 0: the dealiased Galerkin system conserves kinetic energy; RK4 adds
+    an O(dt^5) error per step (far below 1e-12 at dt = 1e-3)."""
+    import dfft_ns
+    n = 256
+    s = _solver(n, 2, 2, dfft_ns.DIVERGENCE, 0.0, 1e-3)
+    s.init_taylor_green(1.0)
+    s.step(3, use_graph=True)
+    assert abs(s.energy() - np.pi ** 3) < 1e-12 * np.pi ** 3
+    assert _max_divergence(n, _gather(s, s.get_state(), 1)) < 1e-12
+    s.destroy()
