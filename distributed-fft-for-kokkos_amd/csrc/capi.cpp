@@ -10,6 +10,7 @@
 #include "group.hpp"
 #include "navier_stokes.hpp"
 #include "plan.hpp"
+#include "trace.hpp"
 #include "planner.hpp"
 
 struct dfft_plan {
@@ -45,6 +46,9 @@ template <typename F> int guarded(F&& f) {
 }
 
 void need(bool ok, const char* what) {
+  if (!ok) throw dfft::Error(DFFT_ERROR_INVALID_ARGUMENT, what);
+}
+void need(bool ok, const std::string& what) {
   if (!ok) throw dfft::Error(DFFT_ERROR_INVALID_ARGUMENT, what);
 }
 
@@ -221,32 +225,51 @@ int dfft_plan_autotune(dfft_plan* plan, void* const* in, void* const* out, void*
   });
 }
 
+namespace {
+
+// Pack/unpack arguments of one rank's local array and its exchange buffer;
+// `layout` is the view layout: LayoutRight buffers are (nprocs, e_0..e_{D-1}),
+// LayoutLeft buffers (e_0..e_{D-1}, nprocs) (Extents.hpp:44-60,
+// PackUnpack.hpp:176-178).
+dfft::PackArgs pack_args(const size_t* arr_extents, const size_t* arr_map, int rank, const size_t* buffer_extents,
+                         int nprocs, int axis, int layout, const char* what) {
+  need(arr_extents && arr_map && buffer_extents, "null pointer");
+  need(rank >= 2 && rank <= DFFT_MAX_RANK, std::string(what) + ": View rank must be in [2, 4]");
+  need(axis >= 0 && axis < rank, "axis out of range");
+  need(nprocs >= 1, "nprocs must be positive");
+  need(layout == DFFT_LAYOUT_RIGHT || layout == DFFT_LAYOUT_LEFT, "layout must be DFFT_LAYOUT_RIGHT or LEFT");
+  const bool left = layout == DFFT_LAYOUT_LEFT;
+  const int pdim = left ? rank : 0;
+  if (static_cast<int>(buffer_extents[pdim]) != nprocs)
+    throw dfft::Error(DFFT_ERROR_MISMATCH, std::string("buffer extent ") + std::to_string(pdim) +
+                                               " must equal the number of processes");
+  dfft::PackArgs a{};
+  a.ndim = rank;
+  a.layout_left = left ? 1 : 0;
+  a.nprocs = nprocs;
+  a.axis = axis;
+  std::size_t pos = dfft::get_index(vec(arr_map, rank), static_cast<std::size_t>(axis));
+  a.merged_size = static_cast<int32_t>(arr_extents[pos]);
+  a.total = 1;
+  for (int k = 0; k < rank; ++k) {
+    a.map[k] = static_cast<int32_t>(arr_map[k]);
+    a.local_ext[k] = static_cast<int64_t>(arr_extents[k]);
+  }
+  for (int k = 0; k <= rank; ++k) {
+    a.buf_ext[k] = static_cast<int64_t>(buffer_extents[k]);
+    a.total *= a.buf_ext[k];
+  }
+  return a;
+}
+
+}  // namespace
+
 int dfft_pack(const void* src, const size_t* src_extents, const size_t* src_map, int rank, void* buffer,
-              const size_t* buffer_extents, int nprocs, int axis, int elem_size, void* stream) {
+              const size_t* buffer_extents, int nprocs, int axis, int layout, int elem_size, void* stream) {
   return guarded([&] {
-    need(src_extents && src_map && buffer_extents, "null pointer");
-    need(rank >= 2 && rank <= DFFT_MAX_RANK, "pack: SrcView rank must be in [2, 4]");
-    need(axis >= 0 && axis < rank, "axis out of range");
-    need(nprocs >= 1, "nprocs must be positive");
-    if (static_cast<int>(buffer_extents[0]) != nprocs)
-      throw dfft::Error(DFFT_ERROR_MISMATCH, "buffer extent 0 must equal the number of processes");
-    dfft::PackArgs a{};
-    a.ndim = rank;
-    a.nprocs = nprocs;
-    a.axis = axis;
-    std::size_t pos = dfft::get_index(vec(src_map, rank), static_cast<std::size_t>(axis));
-    a.merged_size = static_cast<int32_t>(src_extents[pos]);
-    a.total = 1;
+    dfft::PackArgs a = pack_args(src_extents, src_map, rank, buffer_extents, nprocs, axis, layout, "pack");
     std::size_t nsrc = 1;
-    for (int k = 0; k < rank; ++k) {
-      a.map[k] = static_cast<int32_t>(src_map[k]);
-      a.local_ext[k] = static_cast<int64_t>(src_extents[k]);
-      nsrc *= src_extents[k];
-    }
-    for (int k = 0; k <= rank; ++k) {
-      a.buf_ext[k] = static_cast<int64_t>(buffer_extents[k]);
-      a.total *= a.buf_ext[k];
-    }
+    for (int k = 0; k < rank; ++k) nsrc *= src_extents[k];
     need(a.total == 0 || buffer, "null buffer");
     need(nsrc == 0 || src, "null source");
     // TransBlock::good (TransBlock.hpp:175-185): no aliasing between view and buffer
@@ -254,39 +277,21 @@ int dfft_pack(const void* src, const size_t* src_extents, const size_t* src_map,
       throw dfft::Error(DFFT_ERROR_ALIAS, "Input view must not be aliasing with send buffer");
     a.src = static_cast<const char*>(src);
     a.dst = static_cast<char*>(buffer);
+    dfft::TraceRange range("dfft::pack");
     hip_ok(dfft::launch_pack(a, elem_size, false, static_cast<hipStream_t>(stream)), "pack");
   });
 }
 
 int dfft_unpack(const void* buffer, const size_t* buffer_extents, int nprocs, void* dst,
-                const size_t* dst_extents, const size_t* dst_map, int rank, int axis, int elem_size,
+                const size_t* dst_extents, const size_t* dst_map, int rank, int axis, int layout, int elem_size,
                 void* stream) {
   return guarded([&] {
-    need(dst_extents && dst_map && buffer_extents, "null pointer");
-    need(rank >= 2 && rank <= DFFT_MAX_RANK, "unpack: DstView rank must be in [2, 4]");
-    need(axis >= 0 && axis < rank, "axis out of range");
-    need(nprocs >= 1, "nprocs must be positive");
-    if (static_cast<int>(buffer_extents[0]) != nprocs)
-      throw dfft::Error(DFFT_ERROR_MISMATCH, "buffer extent 0 must equal the number of processes");
-    dfft::PackArgs a{};
-    a.ndim = rank;
-    a.nprocs = nprocs;
-    a.axis = axis;
-    std::size_t pos = dfft::get_index(vec(dst_map, rank), static_cast<std::size_t>(axis));
-    a.merged_size = static_cast<int32_t>(dst_extents[pos]);
-    a.total = 1;
-    for (int k = 0; k < rank; ++k) {
-      a.map[k] = static_cast<int32_t>(dst_map[k]);
-      a.local_ext[k] = static_cast<int64_t>(dst_extents[k]);
-    }
-    for (int k = 0; k <= rank; ++k) {
-      a.buf_ext[k] = static_cast<int64_t>(buffer_extents[k]);
-      a.total *= a.buf_ext[k];
-    }
+    dfft::PackArgs a = pack_args(dst_extents, dst_map, rank, buffer_extents, nprocs, axis, layout, "unpack");
     if (dst == buffer && a.total > 0)
       throw dfft::Error(DFFT_ERROR_ALIAS, "Output view must not be aliasing with receive buffer");
     a.src = static_cast<const char*>(buffer);
     a.dst = static_cast<char*>(dst);
+    dfft::TraceRange range("dfft::unpack");
     hip_ok(dfft::launch_pack(a, elem_size, true, static_cast<hipStream_t>(stream)), "unpack");
   });
 }
@@ -305,6 +310,7 @@ int dfft_all2all(void* const* send, void* const* recv, int nprocs, size_t block_
       a.send[p] = static_cast<const char*>(send[p]);
       a.recv[p] = static_cast<char*>(recv[p]);
     }
+    dfft::TraceRange range("dfft::all2all");
     hip_ok(dfft::launch_all2all(a, static_cast<hipStream_t>(stream)), "all2all");
   });
 }
@@ -364,6 +370,7 @@ int dfft_exchange(void* const* in, void* const* out, int rank, const size_t* glo
       for (std::size_t q = 0; q < P; ++q)
         if (in[r] == out[q] && in[r] != nullptr)
           throw dfft::Error(DFFT_ERROR_ALIAS, "input and output local arrays must not alias");
+    dfft::TraceRange range("dfft::exchange (pack + all2all + unpack)");
     hip_ok(dfft::launch_remap(ra, elem_size, static_cast<hipStream_t>(stream)), "exchange");
   });
 }
@@ -637,12 +644,41 @@ int dfft_block_analyses(const dfft_desc* desc, int vrank, dfft_block_info* infos
       throw dfft::Error(DFFT_ERROR_INVALID_TOPOLOGY, "block analyses need slab or pencil topologies");
     need(vrank >= 0 && static_cast<std::size_t>(vrank) < dfft::total_size(ti), "rank out of range");
     dfft::Vec local = dfft::compute_local_extents(gin, ti, dfft::identity_map(rank), vrank, il);
-    auto res = dfft::block_analyses(kind, desc->transform == DFFT_R2C, dfft::Layout::Right, local, gin, gout, ti,
-                                    to, ax, static_cast<std::size_t>(vrank), il, ol);
+    auto vl = desc->layout == DFFT_LAYOUT_LEFT ? dfft::Layout::Left : dfft::Layout::Right;
+    auto res = dfft::block_analyses(kind, desc->transform == DFFT_R2C, vl, local, gin, gout, ti, to, ax,
+                                    static_cast<std::size_t>(vrank), il, ol);
     need(static_cast<int>(res.blocks.size()) <= *nblocks, "output capacity too small");
     for (std::size_t i = 0; i < res.blocks.size(); ++i) fill_info(res.blocks[i], &infos[i]);
     *nblocks = static_cast<int>(res.blocks.size());
     if (max_buffer_size) *max_buffer_size = res.max_buffer_size;
+  });
+}
+
+int dfft_rank_to_coord(int rank, const size_t* topology, int topology_layout, int vrank, size_t* coord) {
+  return guarded([&] {
+    need(topology && coord, "null pointer");
+    need(rank >= 1 && rank <= DFFT_MAX_RANK, "rank must be in [1, 4]");
+    need(vrank >= 0, "rank must be non-negative");
+    auto tl = topology_layout == DFFT_LAYOUT_LEFT ? dfft::Layout::Left : dfft::Layout::Right;
+    dfft::Vec c = dfft::rank_to_coord(vec(topology, rank), static_cast<std::size_t>(vrank), tl);
+    for (int k = 0; k < rank; ++k) coord[k] = c[k];
+  });
+}
+
+int dfft_compute_local_extents_and_starts(int rank, const size_t* global_extents, const size_t* topology,
+                                          int topology_layout, int vrank, size_t* extents, size_t* starts) {
+  return guarded([&] {
+    need(global_extents && topology && extents && starts, "null pointer");
+    need(rank >= 1 && rank <= DFFT_MAX_RANK, "rank must be in [1, 4]");
+    need(vrank >= 0, "rank must be non-negative");
+    auto tl = topology_layout == DFFT_LAYOUT_LEFT ? dfft::Layout::Left : dfft::Layout::Right;
+    dfft::Vec local, st;
+    dfft::compute_local_extents_and_starts(vec(global_extents, rank), vec(topology, rank),
+                                           static_cast<std::size_t>(vrank), tl, local, st);
+    for (int k = 0; k < rank; ++k) {
+      extents[k] = local[k];
+      starts[k] = st[k];
+    }
   });
 }
 

@@ -414,11 +414,23 @@ template <typename E, bool UNPACK>
 __global__ void __launch_bounds__(256) pack_kernel(const PackArgs a) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < a.total; g += stride) {
+    // j[0] = p, j[1 + i] = buffer index along logical axis i; a LayoutRight
+    // buffer is (P, e_0..e_{D-1}) row-major, a LayoutLeft one (e_0..e_{D-1}, P)
+    // column-major (buf_ext in that order)
     int64_t j[kMaxDim + 1];
     int64_t rem = g;
-    for (int k = a.ndim; k >= 0; --k) {
-      j[k] = rem % a.buf_ext[k];
-      rem /= a.buf_ext[k];
+    if (!a.layout_left) {
+      for (int k = a.ndim; k >= 0; --k) {
+        j[k] = rem % a.buf_ext[k];
+        rem /= a.buf_ext[k];
+      }
+    } else {
+      for (int k = 0; k <= a.ndim; ++k) {
+        const int64_t v = rem % a.buf_ext[k];
+        rem /= a.buf_ext[k];
+        if (k < a.ndim) j[k + 1] = v;
+        else j[0] = v;
+      }
     }
     const int64_t p = j[0];
     const int64_t base = a.merged_size / a.nprocs, r = a.merged_size % a.nprocs;
@@ -434,8 +446,11 @@ __global__ void __launch_bounds__(256) pack_kernel(const PackArgs a) {
         l[i] = j[i + 1];
       }
     }
+    // storage offset in the local array (storage axis k holds logical axis
+    // map[k]; row-major for LayoutRight, column-major for LayoutLeft)
     int64_t off = 0;
-    for (int k = 0; k < a.ndim; ++k) {
+    for (int i = 0; i < a.ndim; ++i) {
+      const int k = a.layout_left ? a.ndim - 1 - i : i;
       const int64_t v = l[a.map[k]];
       if (v >= a.local_ext[k]) ok = false;
       off = off * a.local_ext[k] + v;

@@ -101,16 +101,18 @@ bool line_length_supported(int n, bool fp64);
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
 hipError_t launch_remap(const RemapArgs& a, int esize, hipStream_t s);
 
-// Reference-exact pack/unpack of one rank's local array (mapped by `map`,
-// LayoutRight) into/out of a (P, e_0..e_{D-1}) buffer.
+// Reference-exact pack/unpack of one rank's local array (mapped by `map`)
+// into/out of a (P, e_0..e_{D-1}) LayoutRight buffer, or, with layout_left,
+// a LayoutLeft array into/out of an (e_0..e_{D-1}, P) LayoutLeft buffer.
 struct PackArgs {
   int32_t ndim;
+  int32_t layout_left;
   int32_t nprocs;
   int32_t axis;           // logical axis split (pack) / merged (unpack)
   int32_t merged_size;    // local extent of that axis
   int32_t map[kMaxDim];
   int64_t local_ext[kMaxDim];   // storage extents of the local array
-  int64_t buf_ext[kMaxDim + 1]; // (P, e_0..e_{D-1})
+  int64_t buf_ext[kMaxDim + 1]; // (P, e_0..e_{D-1}), or (e_0..e_{D-1}, P) with layout_left
   int64_t total;                // elements of the buffer
   const char* src;
   char* dst;

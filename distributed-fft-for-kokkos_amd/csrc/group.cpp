@@ -6,6 +6,7 @@
 
 #include "buffer_check.hpp"
 #include "plan.hpp"
+#include "trace.hpp"
 
 namespace dfft {
 
@@ -279,6 +280,7 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
 }
 
 void Group::launch_all(int dir, hipStream_t s) {
+  TraceRange range("dfft::Group %s (%zu members)", dir == 0 ? "forward" : "backward", members_.size());
   const DirState& ds = dir_[dir];
   for (int k = 0; k < max_passes_; ++k) {
     for (int v = 0; v < 2; ++v) {

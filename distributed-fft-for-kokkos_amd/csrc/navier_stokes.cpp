@@ -3,6 +3,7 @@
 #include "navier_stokes.hpp"
 
 #include "buffer_check.hpp"
+#include "trace.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -157,6 +158,7 @@ void NavierStokes::one_step(hipStream_t s) {
 }
 
 void NavierStokes::step(int nsteps, bool use_graph, hipStream_t s) {
+  TraceRange range("dfft::NavierStokes::step x%d%s", nsteps, use_graph ? " (graph)" : "");
   if (nsteps <= 0) return;
   if (!use_graph) {
     for (int i = 0; i < nsteps; ++i) one_step(s);

@@ -11,6 +11,7 @@
 //    PencilPlan.hpp:1444-1598): FFT blocks on each rank's mapped local array
 //    and one exchange remap per transpose block.
 #include "plan.hpp"
+#include "trace.hpp"
 
 #include "buffer_check.hpp"
 
@@ -61,7 +62,7 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
     throw Error(DFFT_ERROR_INVALID_ARGUMENT, "precision must be DFFT_FP32 or DFFT_FP64");
   if (d.norm < DFFT_NORM_BACKWARD || d.norm > DFFT_NORM_NONE)
     throw Error(DFFT_ERROR_INVALID_ARGUMENT, "invalid normalization");
-  if (d.exec_mode != DFFT_EXEC_FUSED && d.exec_mode != DFFT_EXEC_BLOCKWALK)
+  if (d.exec_mode != DFFT_EXEC_FUSED && d.exec_mode != DFFT_EXEC_BLOCKWALK && d.exec_mode != DFFT_EXEC_VENDOR)
     throw Error(DFFT_ERROR_INVALID_ARGUMENT, "invalid execution mode");
   rank_ = static_cast<std::size_t>(d.rank);
   real_ = d.transform == DFFT_R2C;
@@ -113,7 +114,7 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
 
   if (kind_ == TopologyType::Slab || kind_ == TopologyType::Pencil) {
     for (std::size_t r = 0; r < P_; ++r) {
-      analyses_.push_back(block_analyses(kind_, real_, Layout::Right, in_local_[r], gin_, gout_, in_topo_,
+      analyses_.push_back(block_analyses(kind_, real_, view_layout_, in_local_[r], gin_, gout_, in_topo_,
                                          out_topo_, axes_, r, in_tl_, out_tl_));
       max_buffer_size_ = std::max(max_buffer_size_, analyses_.back().max_buffer_size);
     }
@@ -123,13 +124,14 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
       throw Error(DFFT_ERROR_INVALID_TOPOLOGY, "Number blocks must be in [1, " + std::to_string(maxb) + "]");
   }
   if (d.exec_mode == DFFT_EXEC_BLOCKWALK) {
-    if (view_layout_ != Layout::Right)
-      throw Error(DFFT_ERROR_UNSUPPORTED, "block-walk execution covers LayoutRight views");
     if (kind_ == TopologyType::Shared)
       throw Error(DFFT_ERROR_UNSUPPORTED, "block-walk execution needs a slab or pencil plan");
   }
 
-  for (std::size_t k = 0; k < axes_.size(); ++k) {
+  // the TplPlan role (TplPlan.hpp:25-181): the vendor library names the plan
+  if (d.exec_mode == DFFT_EXEC_VENDOR) name_ = "rocFFTPlan";
+
+  for (std::size_t k = 0; k < axes_.size() && d.exec_mode != DFFT_EXEC_VENDOR; ++k) {
     const std::size_t n = fft_extents_[k];
     if (!line_length_supported(static_cast<int>(n), fp64_))
       throw Error(DFFT_ERROR_UNSUPPORTED, "no line kernel for transform length " + std::to_string(n) +
@@ -147,6 +149,7 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
 }
 
 void Plan::release() {
+  vendor_.reset();
   for (auto& kv : tw_) (void)hipFree(kv.second);
   tw_.clear();
   if (scratch_) (void)hipFree(scratch_);
@@ -160,6 +163,11 @@ void Plan::release() {
 // device memory of the plan: twiddle tables, then the fused executor's
 // scratch array or the block walk's ping-pong buffers
 void Plan::allocate() {
+  if (d_.exec_mode == DFFT_EXEC_VENDOR) {
+    vendor_ = std::make_unique<VendorFFT>(gin_, gout_, axes_, real_, fp64_, norm_scale(true), norm_scale(false));
+    workspace_bytes_ += vendor_->workspace_bytes();
+    return;
+  }
   // twiddle tables exp(-2 pi i e / n), evaluated in long double and rounded
   // once to FP64 -- for both precisions: the kernels form the powers of a
   // butterfly in FP64 and round each once to the compute precision
@@ -489,6 +497,8 @@ void Plan::run_fused(void* const* in, void* const* out, bool forward, hipStream_
     DDesc a = first ? src : scr;
     DDesc b = last ? (alias ? scratch_desc(forward ? gout_ : gin_, scratch_, alias_es) : dst) : scr;
     LinePass p = make_pass(passes[i], a, b, !forward, last ? scale : 1.0);
+    TraceRange range("dfft::%s pass %zu axis %d%s", forward ? "forward" : "backward", i, passes[i].axis,
+                     first ? " (gather from the input ranks)" : last ? " (scatter to the output ranks)" : "");
     mark(stream);
     check_hip(launch_line_pass(p, fp64_, stream), "line pass");
     mark(stream);
@@ -526,6 +536,14 @@ void Plan::execute(void* const* in, void* const* out, int direction, hipStream_t
   check_hip(hipGetDevice(&cur), "hipGetDevice");
   if (cur != device_) check_hip(hipSetDevice(device_), "hipSetDevice");
   if (total_size(gin_) == 0) return;
+  TraceRange range("dfft::%s::execute %s", name_.c_str(), forward ? "forward" : "backward");
+  if (d_.exec_mode == DFFT_EXEC_VENDOR) {
+    if (adj_scale_ >= 0.0) throw Error(DFFT_ERROR_UNSUPPORTED, "the vendor plan has no adjoint");
+    DDesc src = forward ? user_desc(gin_, in_topo_, in_tl_, in) : user_desc(gout_, out_topo_, out_tl_, in);
+    DDesc dst = forward ? user_desc(gout_, out_topo_, out_tl_, out) : user_desc(gin_, in_topo_, in_tl_, out);
+    vendor_->run(src, dst, forward, stream, [this](hipStream_t s) { mark(s); });
+    return;
+  }
   if (d_.exec_mode == DFFT_EXEC_BLOCKWALK) run_blockwalk(in, out, forward, stream);
   else run_fused(in, out, forward, stream);
 }
@@ -786,6 +804,7 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
 }
 
 int Plan::num_passes(int direction) const {
+  if (d_.exec_mode == DFFT_EXEC_VENDOR) return 3;  // gather, vendor call, scatter
   if (d_.exec_mode == DFFT_EXEC_FUSED) {
     const auto& p = direction == DFFT_FORWARD ? fwd_ : bwd_;
     return static_cast<int>(p.size());
@@ -826,6 +845,7 @@ void Plan::run_blockwalk(void* const* in, void* const* out, bool forward, hipStr
       ra.iter_div[k] = FastDiv(static_cast<uint32_t>(to.ext[a]));
       ra.total *= static_cast<int64_t>(to.ext[a]);
     }
+    TraceRange range("dfft::TransBlock (pack + all2all + unpack)");
     mark(stream);
     check_hip(launch_remap(ra, from.is_real ? static_cast<int>(rs) : static_cast<int>(cs), stream), "remap");
     mark(stream);
@@ -858,8 +878,15 @@ void Plan::run_blockwalk(void* const* in, void* const* out, bool forward, hipStr
   Vec ident = identity_map(rank_);
 
   auto fft_block = [&](State& st, const BlockInfo& b, bool fwd) {
-    // axes of the block: the innermost fft_dim storage axes of the mapped array
-    Vec ax(b.in_map.end() - static_cast<long>(b.fft_dim), b.in_map.end());
+    // axes of the block: the innermost fft_dim storage axes of the mapped
+    // array -- the last ones of a LayoutRight array, the first ones of a
+    // LayoutLeft array -- listed so that the contiguous (R2C) axis is last
+    Vec ax;
+    if (view_layout_ == Layout::Right) {
+      ax.assign(b.in_map.end() - static_cast<long>(b.fft_dim), b.in_map.end());
+    } else {
+      for (std::size_t k = b.fft_dim; k-- > 0;) ax.push_back(b.in_map[k]);
+    }
     const bool r2c = real_ && b.block_idx == 0;
     std::vector<std::size_t> order;
     if (fwd) {
@@ -882,6 +909,7 @@ void Plan::run_blockwalk(void* const* in, void* const* out, bool forward, hipStr
       }
       LinePass p = make_pass(s, desc_of(st), desc_of(dst), !fwd, scaled ? 1.0 : scale);
       scaled = true;
+      TraceRange range("dfft::FFT block %zu axis %zu", static_cast<std::size_t>(b.block_idx), a);
       mark(stream);
       check_hip(launch_line_pass(p, fp64_, stream), "line pass");
       mark(stream);

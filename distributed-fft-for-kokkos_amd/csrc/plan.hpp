@@ -6,12 +6,14 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "dfft.h"
 #include "fft_kernels.hpp"
 #include "planner.hpp"
+#include "vendor.hpp"
 
 namespace dfft {
 
@@ -107,6 +109,7 @@ class Plan {
   std::size_t max_buffer_size_ = 0;
   void* scratch_ = nullptr;
   std::vector<void*> walk_bufs_[2];  // block-walk ping-pong local arrays per rank
+  std::unique_ptr<VendorFFT> vendor_;  // DFFT_EXEC_VENDOR: rocFFT over the gathered global array
   std::size_t workspace_bytes_ = 0;
   std::map<int, void*> tw_;
   std::vector<PassSpec> fwd_, bwd_;

@@ -49,7 +49,7 @@ C2C, R2C = 0, 1
 FP32, FP64 = 0, 1
 LAYOUT_RIGHT, LAYOUT_LEFT = 0, 1
 NORMS = {"backward": 0, "forward": 1, "ortho": 2, "none": 3}
-EXEC_FUSED, EXEC_BLOCKWALK = 0, 1
+EXEC_FUSED, EXEC_BLOCKWALK, EXEC_VENDOR = 0, 1, 2
 
 
 class Direction(enum.IntEnum):
@@ -160,7 +160,8 @@ EXPORTED = [
     "dfft_plan_max_buffer_size", "dfft_plan_workspace_bytes", "dfft_execute", "dfft_execute_adjoint",
     "dfft_plan_num_passes",
     "dfft_execute_timed", "dfft_plan_set_option", "dfft_plan_get_option", "dfft_plan_autotune", "dfft_pack",
-    "dfft_unpack", "dfft_all2all", "dfft_exchange", "dfft_get_all_slab_topologies",
+    "dfft_unpack", "dfft_all2all", "dfft_exchange", "dfft_get_all_slab_topologies", "dfft_rank_to_coord",
+    "dfft_compute_local_extents_and_starts",
     "dfft_get_all_pencil_topologies", "dfft_block_analyses", "dfft_group_create", "dfft_group_destroy",
     "dfft_group_execute", "dfft_group_num_launches", "dfft_group_set_option", "dfft_group_get_option",
     "dfft_group_autotune", "dfft_ns_create", "dfft_ns_destroy",
@@ -208,15 +209,18 @@ def load_library(path: os.PathLike | None = None):
         "dfft_plan_get_option": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "dfft_plan_autotune": ([ctypes.c_void_p, vpp, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_pack": ([ctypes.c_void_p, szp, szp, ctypes.c_int, ctypes.c_void_p, szp, ctypes.c_int,
-                       ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
         "dfft_unpack": ([ctypes.c_void_p, szp, ctypes.c_int, ctypes.c_void_p, szp, szp, ctypes.c_int,
-                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
         "dfft_all2all": ([vpp, vpp, ctypes.c_int, sz, ctypes.c_void_p], ctypes.c_int),
         "dfft_exchange": ([vpp, vpp, ctypes.c_int, szp, szp, ctypes.c_int, szp, szp, ctypes.c_int, szp,
                            ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
         "dfft_get_all_slab_topologies": ([ctypes.c_int, szp, szp, ctypes.c_int, ip, szp, ip], ctypes.c_int),
         "dfft_get_all_pencil_topologies": ([ctypes.c_int, szp, ctypes.c_int, szp, ctypes.c_int, ctypes.c_int,
                                             ip, szp, szp, szp, ip], ctypes.c_int),
+        "dfft_rank_to_coord": ([ctypes.c_int, szp, ctypes.c_int, ctypes.c_int, szp], ctypes.c_int),
+        "dfft_compute_local_extents_and_starts": ([ctypes.c_int, szp, szp, ctypes.c_int, ctypes.c_int, szp, szp],
+                                                  ctypes.c_int),
         "dfft_block_analyses": ([ctypes.POINTER(_Desc), ctypes.c_int, ctypes.POINTER(BlockInfo), ip, szp],
                                 ctypes.c_int),
         "dfft_group_create": ([ctypes.c_int, ctypes.POINTER(_Desc), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
@@ -345,10 +349,11 @@ def _make_desc(gin, gout, axes, in_topology, out_topology, real, fp64, norm, exe
     return d
 
 
-def block_analyses(gin, gout, axes, in_topology, out_topology, rank, real=False):
-    """Block list of one virtual rank (BlockAnalyses.hpp:415-506)."""
+def block_analyses(gin, gout, axes, in_topology, out_topology, rank, real=False, layout=LayoutRight):
+    """Block list of one virtual rank (BlockAnalyses.hpp:415-506) for views of ``layout``."""
     lib = load_library()
-    d = _make_desc(gin, gout, axes, in_topology, out_topology, real, True, "backward", EXEC_FUSED)
+    d = _make_desc(gin, gout, axes, in_topology, out_topology, real, True, "backward", EXEC_FUSED,
+                   _layout_value(layout))
     infos = (BlockInfo * 16)()
     nb = ctypes.c_int(16)
     mx = ctypes.c_size_t(0)
@@ -357,29 +362,26 @@ def block_analyses(gin, gout, axes, in_topology, out_topology, rank, real=False)
 
 
 def rank_to_coord(topology, rank):
-    """MPI_Extents.hpp:492-529."""
+    """MPI_Extents.hpp:492-529 (``dfft_rank_to_coord``)."""
     t = _as_topology(topology)
-    if rank >= _prod(t):
-        raise DFFTError(ERROR_INVALID_ARGUMENT, "rank must be less than topology size.")
-    coord = [0] * len(t)
-    r = int(rank)
-    order = range(len(t) - 1, -1, -1) if t.layout is LayoutRight else range(len(t))
-    for i in order:
-        coord[i] = r % t[i]
-        r //= t[i]
-    return coord
+    lib = load_library()
+    c = (ctypes.c_size_t * len(t))()
+    _check(lib.dfft_rank_to_coord(len(t), _sizes(t), t.layout.value, int(rank), c))
+    return [int(v) for v in c]
 
 
 def compute_local_extents_and_starts(extents, topology, rank):
-    """MPI_Extents.hpp:546-577 for virtual rank ``rank``."""
+    """MPI_Extents.hpp:546-577 for virtual rank ``rank``
+    (``dfft_compute_local_extents_and_starts``)."""
     t = _as_topology(topology)
-    c = rank_to_coord(t, rank)
-    ext, st = [], []
-    for i, n in enumerate(extents):
-        q, r = divmod(int(n), int(t[i]))
-        ext.append(q + 1 if c[i] < r else q)
-        st.append(c[i] * q + min(c[i], r))
-    return ext, st
+    if len(extents) != len(t):
+        raise DFFTError(ERROR_INVALID_ARGUMENT, "extents and topology must have the same rank")
+    lib = load_library()
+    e = (ctypes.c_size_t * len(t))()
+    st = (ctypes.c_size_t * len(t))()
+    _check(lib.dfft_compute_local_extents_and_starts(len(t), _sizes(extents), _sizes(t), t.layout.value, int(rank),
+                                                     e, st))
+    return [int(v) for v in e], [int(v) for v in st]
 
 
 def _prod(v):
@@ -734,23 +736,48 @@ def execute(plan: Plan, in_views, out_views, direction, stream=None):
 # Exchange layer
 # ---------------------------------------------------------------------------
 
-def pack(src, buffer, src_map, axis, stream=None):
-    """``pack(exec, src, dst, src_map, axis)`` (PackUnpack.hpp:439-460)."""
+def _layout_value(layout):
+    return layout.value if hasattr(layout, "value") else int(layout)
+
+
+def _check_exchange_views(view, buffer, layout):
+    for what, t in (("view", view), ("buffer", buffer)):
+        if t.numel() and not _is_layout_contiguous(t, layout):
+            raise DFFTError(ERROR_INVALID_ARGUMENT, f"the {what} must be contiguous in the "
+                            f"{'LayoutLeft' if layout == LAYOUT_LEFT else 'LayoutRight'} order")
+        if t.numel() and not t.is_cuda:
+            raise DFFTError(ERROR_INVALID_ARGUMENT, f"the {what} must be a device array")
+    if view.dtype != buffer.dtype:
+        raise DFFTError(ERROR_MISMATCH, "view and buffer value types differ")
+
+
+def pack(src, buffer, src_map, axis, stream=None, layout=LayoutRight):
+    """``pack(exec, src, dst, src_map, axis)`` (PackUnpack.hpp:439-460).
+
+    ``layout``: the views' layout.  LayoutRight buffers are (P, e_0..e_{D-1});
+    LayoutLeft ones (e_0..e_{D-1}, P), both given by their logical extents
+    (a LayoutLeft tensor is column-major: strides increase along the axes)."""
     lib = load_library()
+    lay = _layout_value(layout)
+    _check_exchange_views(src, buffer, lay)
     rank = src.dim()
+    nprocs = int(buffer.shape[-1] if lay == LAYOUT_LEFT else buffer.shape[0])
     _check(lib.dfft_pack(ctypes.c_void_p(src.data_ptr() if src.numel() else None), _sizes(src.shape),
                          _sizes(src_map), rank, ctypes.c_void_p(buffer.data_ptr() if buffer.numel() else None),
-                         _sizes(buffer.shape), int(buffer.shape[0]), int(axis), src.element_size(),
+                         _sizes(buffer.shape), nprocs, int(axis), lay, src.element_size(),
                          _stream_handle(stream)))
 
 
-def unpack(buffer, dst, dst_map, axis, stream=None):
+def unpack(buffer, dst, dst_map, axis, stream=None, layout=LayoutRight):
     """``unpack(exec, src, dst, dst_map, axis)`` (PackUnpack.hpp:478-499)."""
     lib = load_library()
+    lay = _layout_value(layout)
+    _check_exchange_views(dst, buffer, lay)
     rank = dst.dim()
+    nprocs = int(buffer.shape[-1] if lay == LAYOUT_LEFT else buffer.shape[0])
     _check(lib.dfft_unpack(ctypes.c_void_p(buffer.data_ptr() if buffer.numel() else None), _sizes(buffer.shape),
-                           int(buffer.shape[0]), ctypes.c_void_p(dst.data_ptr() if dst.numel() else None),
-                           _sizes(dst.shape), _sizes(dst_map), rank, int(axis), dst.element_size(),
+                           nprocs, ctypes.c_void_p(dst.data_ptr() if dst.numel() else None),
+                           _sizes(dst.shape), _sizes(dst_map), rank, int(axis), lay, dst.element_size(),
                            _stream_handle(stream)))
 
 

@@ -55,6 +55,8 @@ extern "C" {
 /* execution strategies of a plan */
 #define DFFT_EXEC_FUSED 0     /* exchange fused into the FFT passes (default) */
 #define DFFT_EXEC_BLOCKWALK 1 /* the reference's block sequence: FFT blocks + exchanges */
+#define DFFT_EXEC_VENDOR 2    /* rocFFT on the gathered global array (the TplPlan role, TplPlan.hpp:25-181);
+                                 a comparator: no adjoint, plan name "rocFFTPlan" */
 
 #define DFFT_MAX_RANK 4
 #define DFFT_MAX_VRANKS 64
@@ -166,13 +168,17 @@ int dfft_plan_autotune(dfft_plan* plan, void* const* in, void* const* out, void*
 
 /* ---- exchange layer (standalone, one GPU) ------------------------------ */
 /* pack() (PackUnpack.hpp:439-460) of one rank's local array `src` (storage
- * extents src_extents, mapped by src_map, LayoutRight) into the buffer
- * (nprocs, buffer_extents[0..rank-1]) along logical `axis`. */
+ * extents src_extents, mapped by src_map) into the exchange buffer along
+ * logical `axis`.  `layout` is the views' layout: DFFT_LAYOUT_RIGHT arrays
+ * are row-major and their buffer is (nprocs, e_0..e_{rank-1});
+ * DFFT_LAYOUT_LEFT arrays are column-major and their buffer is
+ * (e_0..e_{rank-1}, nprocs) (PackUnpack.hpp:176-178, Extents.hpp:44-60).
+ * buffer_extents (rank + 1 values) are given in that order. */
 int dfft_pack(const void* src, const size_t* src_extents, const size_t* src_map, int rank, void* buffer,
-              const size_t* buffer_extents, int nprocs, int axis, int elem_size, void* stream);
+              const size_t* buffer_extents, int nprocs, int axis, int layout, int elem_size, void* stream);
 /* unpack() (PackUnpack.hpp:478-499): the inverse placement into `dst`. */
 int dfft_unpack(const void* buffer, const size_t* buffer_extents, int nprocs, void* dst,
-                const size_t* dst_extents, const size_t* dst_map, int rank, int axis, int elem_size,
+                const size_t* dst_extents, const size_t* dst_map, int rank, int axis, int layout, int elem_size,
                 void* stream);
 /* all2all() (MPI_All2All.hpp:23-72) among P virtual ranks: block p of rank
  * r's send buffer lands as block r of rank p's receive buffer. */
@@ -279,6 +285,15 @@ int dfft_get_all_pencil_topologies(int rank, const size_t* in_topology, int in_l
 /* Block analyses of one rank without allocating a plan. */
 int dfft_block_analyses(const dfft_desc* desc, int vrank, dfft_block_info* infos, int* nblocks,
                         size_t* max_buffer_size);
+/* rank_to_coord (MPI_Extents.hpp:492-529): coordinates of rank `vrank` in a
+ * Topology<size_t, rank, topology_layout>; DFFT_ERROR_INVALID_ARGUMENT when
+ * vrank >= the topology size. */
+int dfft_rank_to_coord(int rank, const size_t* topology, int topology_layout, int vrank, size_t* coord);
+/* compute_local_extents_and_starts (MPI_Extents.hpp:546-577): local extents
+ * and global starts of rank `vrank`'s block of a global array distributed
+ * over the topology (first extent % topology bins one element longer). */
+int dfft_compute_local_extents_and_starts(int rank, const size_t* global_extents, const size_t* topology,
+                                          int topology_layout, int vrank, size_t* extents, size_t* starts);
 
 #ifdef __cplusplus
 }

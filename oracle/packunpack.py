@@ -3,10 +3,13 @@ movement on the hot path -- bin mapping, pack, all-to-all, unpack, local
 transpose.  Only the test-suite, ``__graft_entry__.smoke()`` and the CPU
 baseline of ``bench.py`` may use it, and only as the checker.
 
-Arrays are numpy arrays in C order (the reference's LayoutRight).  A local
+Arrays are numpy arrays indexed like the reference's views (the storage
+order of a LayoutLeft view does not change its index semantics).  A local
 array "mapped by m" stores logical axis ``m[k]`` at array axis ``k``.
-Buffers are ``(P, e_0, ..., e_{D-1})`` with the ``e_i`` in *logical* axis
-order, as in ``KokkosFFT_Distributed_PackUnpack.hpp:217-233``.
+LayoutRight buffers are ``(P, e_0, ..., e_{D-1})`` and LayoutLeft buffers
+``(e_0, ..., e_{D-1}, P)``, with the ``e_i`` in *logical* axis order, as in
+``KokkosFFT_Distributed_PackUnpack.hpp:176-178, 217-233`` and
+``KokkosFFT_Distributed_Extents.hpp:44-60``.
 """
 
 from __future__ import annotations
@@ -16,6 +19,13 @@ import itertools
 import numpy as np
 
 OUT_OF_BOUNDS = -1  # PackUnpack.hpp:13-21
+LAYOUT_RIGHT, LAYOUT_LEFT = "right", "left"  # the planner's layout tags
+
+
+def _right_shape(buffer_shape, layout):
+    """(P, e...) shape of a buffer given in its layout's order."""
+    buffer_shape = list(buffer_shape)
+    return buffer_shape if layout == LAYOUT_RIGHT else [buffer_shape[-1]] + buffer_shape[:-1]
 
 
 def bin_mapping(n: int, nbins: int, ibin: int):
@@ -39,12 +49,15 @@ def _logical_view(a: np.ndarray, m):
     return a.transpose(inv)
 
 
-def pack(src: np.ndarray, buffer_shape, src_map, axis: int) -> np.ndarray:
-    """Pack::PackInternal (PackUnpack.hpp:217-233), LayoutRight buffer.
+def pack(src: np.ndarray, buffer_shape, src_map, axis: int, layout=LAYOUT_RIGHT) -> np.ndarray:
+    """Pack::PackInternal (PackUnpack.hpp:217-233); the process index is the
+    buffer's first index (LayoutRight) or last (LayoutLeft, :176-178).
 
     ``merged_size`` = src.extent(index of ``axis`` in ``src_map``) (:446-447).
     Vectorised; ``pack_literal`` is the element-wise form it is tested against.
     """
+    if layout != LAYOUT_RIGHT:
+        return np.moveaxis(pack(src, _right_shape(buffer_shape, layout), src_map, axis), 0, -1)
     nprocs = buffer_shape[0]
     body = list(buffer_shape[1:])
     dst = np.zeros(buffer_shape, dtype=src.dtype)
@@ -67,11 +80,15 @@ def pack(src: np.ndarray, buffer_shape, src_map, axis: int) -> np.ndarray:
     return dst
 
 
-def unpack(buf: np.ndarray, dst_shape, dst_map, axis: int, dst: np.ndarray | None = None) -> np.ndarray:
-    """Unpack::UnpackInternal (PackUnpack.hpp:385-417), LayoutRight buffer.
+def unpack(buf: np.ndarray, dst_shape, dst_map, axis: int, dst: np.ndarray | None = None,
+           layout=LAYOUT_RIGHT) -> np.ndarray:
+    """Unpack::UnpackInternal (PackUnpack.hpp:385-417); LayoutLeft buffers
+    carry the process index last.
 
     Entries of ``dst`` that no buffer element maps to are left untouched
     (zero-initialised when ``dst`` is not given)."""
+    if layout != LAYOUT_RIGHT:
+        return unpack(np.moveaxis(buf, -1, 0), dst_shape, dst_map, axis, dst)
     if dst is None:
         dst = np.zeros(dst_shape, dtype=buf.dtype)
     nprocs = buf.shape[0]
@@ -95,16 +112,19 @@ def unpack(buf: np.ndarray, dst_shape, dst_map, axis: int, dst: np.ndarray | Non
     return dst
 
 
-def pack_literal(src, buffer_shape, src_map, axis):
-    """Element-by-element statement of Pack (PackUnpack.hpp:217-233)."""
+def pack_literal(src, buffer_shape, src_map, axis, layout=LAYOUT_RIGHT):
+    """Element-by-element statement of Pack (PackUnpack.hpp:217-233); the
+    LayoutLeft branch (:236-247) takes p from the last buffer index."""
     dst = np.zeros(buffer_shape, dtype=src.dtype)
-    nprocs = buffer_shape[0]
+    left = layout != LAYOUT_RIGHT
+    nprocs = buffer_shape[-1] if left else buffer_shape[0]
     d = len(src_map)
     merged = src.shape[list(src_map).index(axis)]
     for idx in itertools.product(*[range(n) for n in buffer_shape]):
-        p = idx[0]
+        p = idx[-1] if left else idx[0]
+        body = idx[:-1] if left else idx[1:]
         start, extent = bin_mapping(merged, nprocs, p)
-        l = [merge_indices(idx[i + 1], start, extent, i, axis) for i in range(d)]
+        l = [merge_indices(body[i], start, extent, i, axis) for i in range(d)]
         ok = True
         for k in range(d):
             v = l[src_map[k]]
@@ -116,16 +136,18 @@ def pack_literal(src, buffer_shape, src_map, axis):
     return dst
 
 
-def unpack_literal(buf, dst_shape, dst_map, axis):
+def unpack_literal(buf, dst_shape, dst_map, axis, layout=LAYOUT_RIGHT):
     """Element-by-element statement of Unpack (PackUnpack.hpp:385-417)."""
     dst = np.zeros(dst_shape, dtype=buf.dtype)
-    nprocs = buf.shape[0]
+    left = layout != LAYOUT_RIGHT
+    nprocs = buf.shape[-1] if left else buf.shape[0]
     d = len(dst_map)
     merged = dst_shape[list(dst_map).index(axis)]
     for idx in itertools.product(*[range(n) for n in buf.shape]):
-        p = idx[0]
+        p = idx[-1] if left else idx[0]
+        body = idx[:-1] if left else idx[1:]
         start, extent = bin_mapping(merged, nprocs, p)
-        l = [merge_indices(idx[i + 1], start, extent, i, axis) for i in range(d)]
+        l = [merge_indices(body[i], start, extent, i, axis) for i in range(d)]
         ok = True
         for k in range(d):
             v = l[dst_map[k]]
@@ -137,13 +159,17 @@ def unpack_literal(buf, dst_shape, dst_map, axis):
     return dst
 
 
-def all2all(sends):
-    """MPI_Alltoall over the outermost buffer dimension (MPI_All2All.hpp:54-57):
-    rank r's block p lands at rank p as block r."""
+def all2all(sends, layout=LAYOUT_RIGHT):
+    """MPI_Alltoall over the process dimension of the buffers -- the first
+    (LayoutRight) or last (LayoutLeft) one (MPI_All2All.hpp:44-57): rank r's
+    block p lands at rank p as block r."""
     nprocs = len(sends)
+    pdim = 0 if layout == LAYOUT_RIGHT else -1
     for s in sends:
-        if s.shape[0] != nprocs:
+        if s.shape[pdim] != nprocs:
             raise RuntimeError("Extent of dimension to be transposed must be equal to the number of processes")
+    if layout != LAYOUT_RIGHT:
+        return [np.stack([sends[r][..., p] for r in range(nprocs)], axis=-1) for p in range(nprocs)]
     return [np.stack([sends[r][p] for r in range(nprocs)]) for p in range(nprocs)]
 
 

@@ -35,8 +35,11 @@ class OraclePlan:
     def __init__(self, gin, gout, axes, in_topology, out_topology, real_input=False,
                  norm="backward", layout=PL.LAYOUT_RIGHT, in_layout=PL.LAYOUT_RIGHT,
                  out_layout=PL.LAYOUT_RIGHT):
-        if layout != PL.LAYOUT_RIGHT:
-            raise NotImplementedError("oracle executor covers LayoutRight views")
+        # ``layout``: the views' layout (the reference's LayoutType template
+        # parameter): it selects the block analyses' maps and buffer shapes
+        # and the storage axes the FFT blocks transform; values are indexed
+        # alike for both layouts
+        self.layout = layout
         self.gin = list(gin)
         self.gout = list(gout)
         self.dim = len(gin)
@@ -67,7 +70,7 @@ class OraclePlan:
         self.fft_extents = PL.compute_fft_extents(self.gin, self.gout, self.axes)
         self.analyses = None
         if self.kind in (PL.SLAB, PL.PENCIL):
-            self.analyses = [PL.block_analyses(self.kind, real_input, PL.LAYOUT_RIGHT,
+            self.analyses = [PL.block_analyses(self.kind, real_input, layout,
                                                self.in_local[r], self.gin, self.gout,
                                                self.in_topology, self.out_topology, self.axes, r,
                                                in_layout, out_layout)
@@ -102,20 +105,28 @@ class OraclePlan:
             for r in grp:
                 b = self.analyses[r].blocks[bi]
                 smap, sax = (b.in_map, b.in_axis) if forward else (b.out_map, b.out_axis)
-                if len(grp) != b.buffer_extents[0]:
+                pdim = 0 if self.layout == PL.LAYOUT_RIGHT else -1
+                if len(grp) != b.buffer_extents[pdim]:
                     raise RuntimeError("communicator size differs from the buffer's rank extent")
-                sends.append(PU.pack(state[r], b.buffer_extents, smap, sax))
-            recvs = PU.all2all(sends)
+                sends.append(PU.pack(state[r], b.buffer_extents, smap, sax, self.layout))
+            recvs = PU.all2all(sends, self.layout)
             for r, rb in zip(grp, recvs):
                 b = self.analyses[r].blocks[bi]
                 dmap, dax = (b.out_map, b.out_axis) if forward else (b.in_map, b.in_axis)
                 dshape = b.out_extents if forward else b.in_extents
-                new_state[r] = PU.unpack(rb, dshape, dmap, dax)
+                new_state[r] = PU.unpack(rb, dshape, dmap, dax, layout=self.layout)
         return new_state
 
     def _fft_block(self, a, b, forward):
+        """kokkos-fft DynPlan over the block's fft_dim innermost storage axes:
+        the last ones of a LayoutRight array, the first ones of a LayoutLeft
+        array (listed here so that the R2C axis -- the contiguous one -- is
+        last)."""
         d = self.dim
-        inner = list(range(d - b.fft_dim, d))
+        if self.layout == PL.LAYOUT_RIGHT:
+            inner = list(range(d - b.fft_dim, d))
+        else:
+            inner = list(range(b.fft_dim - 1, -1, -1))
         is_r2c = self.real_input and b.block_idx == 0
         if forward:
             if is_r2c:
@@ -128,7 +139,7 @@ class OraclePlan:
             for ax in inner[:-1]:
                 a = F.c2c(a, ax, +1)
             if is_r2c:
-                a = F.c2r(a, b.in_extents[-1], inner[-1])
+                a = F.c2r(a, b.in_extents[inner[-1]], inner[-1])
             else:
                 a = F.c2c(a, inner[-1], +1)
         return a

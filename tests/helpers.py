@@ -38,11 +38,31 @@ def gather_rel_l2(gpu_locals, ref_locals):
     return float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))
 
 
+def device_array(a, dev, dtype, left=False):
+    """numpy array -> device tensor of the same logical shape, row-major
+    (LayoutRight) or column-major (LayoutLeft: strides grow along the axes)."""
+    import torch
+    if not left:
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev, dtype)
+    rev = list(range(a.ndim))[::-1]
+    return torch.from_numpy(np.ascontiguousarray(a.transpose(rev))).to(dev, dtype).permute(rev)
+
+
+def device_full(shape, value, dtype, dev, left=False):
+    import torch
+    if not left:
+        return torch.full(list(shape), value, dtype=dtype, device=dev)
+    rev = list(range(len(shape)))[::-1]
+    return torch.full(list(shape)[::-1], value, dtype=dtype, device=dev).permute(rev)
+
+
 def run_gpu_plan(dev, x, gin, axes, tin, tout, real, fp64=True, norm="backward", exec_mode=0,
-                 tin_layout=None, tout_layout=None, configure=None):
-    """Forward + backward through the C ABI; returns (outs, backs, plan) as numpy."""
+                 tin_layout=None, tout_layout=None, configure=None, view_layout=None):
+    """Forward + backward through the C ABI; returns (outs, backs, plan) as numpy.
+    view_layout="left": every view is a LayoutLeft (column-major) array."""
     import torch
     import dfft
+    left = view_layout == "left"
 
     tin_t = dfft.Topology(tin, dfft.LayoutLeft if tin_layout == "left" else dfft.LayoutRight)
     tout_t = dfft.Topology(tout, dfft.LayoutLeft if tout_layout == "left" else dfft.LayoutRight)
@@ -51,20 +71,19 @@ def run_gpu_plan(dev, x, gin, axes, tin, tout, real, fp64=True, norm="backward",
     gout = out_extents(gin, axes, real)
     rdt = torch.float64 if fp64 else torch.float32
     cdt = torch.complex128 if fp64 else torch.complex64
-    ins = [torch.from_numpy(np.ascontiguousarray(a)).to(dev, rdt if real else cdt)
-           for a in PE.scatter(x, gin, tin, lin)]
+    ins = [device_array(a, dev, rdt if real else cdt, left) for a in PE.scatter(x, gin, tin, lin)]
     outs = []
     for r in range(len(ins)):
         e, _ = PL.compute_local_extents_and_starts(gout, list(tout), r, lout)
-        outs.append(torch.full(e, float("nan"), dtype=cdt, device=dev))
+        outs.append(device_full(e, float("nan"), cdt, dev, left))
     plan = dfft.Plan(ins, outs, axes, tin_t, tout_t, norm=norm, exec_mode=exec_mode,
-                     global_in_extents=gin, global_out_extents=gout)
+                     global_in_extents=gin, global_out_extents=gout,
+                     layout=dfft.LayoutLeft if left else dfft.LayoutRight)
     if configure is not None:
         configure(plan, ins, outs)
-        ins = [torch.from_numpy(np.ascontiguousarray(a)).to(dev, rdt if real else cdt)
-               for a in PE.scatter(x, gin, tin, lin)]
+        ins = [device_array(a, dev, rdt if real else cdt, left) for a in PE.scatter(x, gin, tin, lin)]
     dfft.execute(plan, ins, outs, dfft.Direction.forward)
-    backs = [torch.full(list(a.shape), float("nan"), dtype=a.dtype, device=dev) for a in ins]
+    backs = [device_full(list(a.shape), float("nan"), a.dtype, dev, left) for a in ins]
     dfft.execute(plan, outs, backs, dfft.Direction.backward)
     torch.cuda.synchronize()
     res = ([o.cpu().numpy() for o in outs], [b.cpu().numpy() for b in backs],

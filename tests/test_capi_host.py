@@ -52,10 +52,10 @@ def test_product_slab_topologies_throw():
     assert e.value.code == dfft.ERROR_INVALID_TOPOLOGY
 
 
-def _oracle_blocks(gin, gout, axes, tin, tout, rank, real):
+def _oracle_blocks(gin, gout, axes, tin, tout, rank, real, layout=PL.LAYOUT_RIGHT):
     kind = PL.get_common_topology_type(tin, tout)
     local = PL.compute_local_extents(gin, tin, list(range(len(gin))), rank)
-    ba = PL.block_analyses(kind, real, PL.LAYOUT_RIGHT, local, gin, gout, tin, tout, axes, rank)
+    ba = PL.block_analyses(kind, real, layout, local, gin, gout, tin, tout, axes, rank)
     out = []
     for b in ba.blocks:
         out.append({"in_topology": b.in_topology, "out_topology": b.out_topology,
@@ -66,9 +66,12 @@ def _oracle_blocks(gin, gout, axes, tin, tout, rank, real):
     return out, ba.max_buffer_size
 
 
-def test_product_block_analyses_match_oracle():
+@pytest.mark.parametrize("layout", [PL.LAYOUT_RIGHT, PL.LAYOUT_LEFT])
+def test_product_block_analyses_match_oracle(layout):
     """Every rank's BlockInfo list of the C++ planner equals the oracle's
-    (slab P=2..4 and pencil 2x2, 2x3 with every pair and axis order, R2C/C2C)."""
+    (slab P=2..4 and pencil 2x2, 2x3 with every pair and axis order, R2C/C2C)
+    for LayoutRight and LayoutLeft views."""
+    lay = dfft.LayoutLeft if layout == PL.LAYOUT_LEFT else dfft.LayoutRight
     n = 0
     for gin in ([8, 7, 5], [9, 6, 7]):
         for P in (2, 3, 4):
@@ -80,8 +83,8 @@ def test_product_block_analyses_match_oracle():
                         if real:
                             gout[axes[-1]] = gin[axes[-1]] // 2 + 1
                         for r in range(P):
-                            exp = _oracle_blocks(gin, gout, list(axes), tin, tout, r, real)
-                            got = dfft.block_analyses(gin, gout, list(axes), tin, tout, r, real)
+                            exp = _oracle_blocks(gin, gout, list(axes), tin, tout, r, real, layout)
+                            got = dfft.block_analyses(gin, gout, list(axes), tin, tout, r, real, lay)
                             assert got == exp, (gin, tin, tout, axes, real, r)
                             n += 1
     for px, py in ((2, 2), (2, 3), (3, 2)):
@@ -94,8 +97,8 @@ def test_product_block_analyses_match_oracle():
                     if real:
                         gout[axes[-1]] = gin[axes[-1]] // 2 + 1
                     for r in range(px * py):
-                        exp = _oracle_blocks(gin, gout, list(axes), tin, tout, r, real)
-                        got = dfft.block_analyses(gin, gout, list(axes), tin, tout, r, real)
+                        exp = _oracle_blocks(gin, gout, list(axes), tin, tout, r, real, layout)
+                        got = dfft.block_analyses(gin, gout, list(axes), tin, tout, r, real, lay)
                         assert got == exp, (gin, tin, tout, axes, real, r)
                         n += 1
     assert n > 3000

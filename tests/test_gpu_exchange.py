@@ -28,34 +28,65 @@ def _global(dtype):
     return g.astype(dtype)
 
 
+def _device(a, dev, layout):
+    """numpy array -> device tensor of the same logical shape, stored
+    row-major (LayoutRight) or column-major (LayoutLeft)."""
+    import torch
+    if layout == PL.LAYOUT_RIGHT:
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    rev = list(range(a.ndim))[::-1]
+    return torch.from_numpy(np.ascontiguousarray(a.transpose(rev))).to(dev).permute(rev)
+
+
+def _filled(shape, value, dtype, dev, layout):
+    import torch
+    if layout == PL.LAYOUT_RIGHT:
+        return torch.full(list(shape), value, dtype=dtype, device=dev)
+    rev = list(range(len(shape)))[::-1]
+    return torch.full(list(shape)[::-1], value, dtype=dtype, device=dev).permute(rev)
+
+
+@pytest.mark.parametrize("layout", [PL.LAYOUT_RIGHT, PL.LAYOUT_LEFT])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64, np.complex128])
-def test_pack_unpack_known_answer(dev, dtype):
+def test_pack_unpack_known_answer(dev, dtype, layout):
+    """LayoutLeft: column-major views and (e0, e1, e2, P) column-major buffers
+    (the float/double x LayoutLeft/LayoutRight types of Test_Pack.cpp:11-14)."""
     import torch
     import dfft
 
+    lay = dfft.LayoutLeft if layout == PL.LAYOUT_LEFT else dfft.LayoutRight
     g = _global(dtype)
-    for nprocs in range(1, 5):
+    for nprocs in range(2, 5):
         topos = [[1, 1, nprocs], [1, nprocs, 1], [nprocs, 1, 1]]
-        for tsrc, tdst in itertools.permutations(topos, 2) if nprocs > 1 else [(topos[0], topos[0])]:
-            if nprocs == 1:
-                continue
+        for tsrc, tdst in itertools.permutations(topos, 2):
             in_axis, out_axis = PL.slab_in_out_axes(tsrc, tdst)
-            bext = PL.compute_buffer_extents(PL.LAYOUT_RIGHT, list(GN), tsrc, tdst)
+            bext = PL.compute_buffer_extents(layout, list(GN), tsrc, tdst)
             for rank in range(nprocs):
                 loc = PE.scatter(g, list(GN), tsrc)[rank]
                 for m in itertools.permutations(range(3)):
                     src = np.ascontiguousarray(loc.transpose(m))
-                    ref = PU.pack(src, bext, list(m), in_axis)
-                    ts = torch.from_numpy(src).to(dev)
-                    tb = torch.full(bext, 7, dtype=ts.dtype, device=dev)
-                    dfft.pack(ts, tb, list(m), in_axis)
-                    got = tb.cpu().numpy()
-                    assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), (nprocs, tsrc, tdst, rank, m)
+                    ref = PU.pack(src, bext, list(m), in_axis, layout)
+                    ts = _device(src, dev, layout)
+                    tb = _filled(bext, 7, ts.dtype, dev, layout)
+                    dfft.pack(ts, tb, list(m), in_axis, layout=lay)
+                    got = np.ascontiguousarray(tb.cpu().numpy())
+                    assert np.array_equal(got.view(np.uint8), np.ascontiguousarray(ref).view(np.uint8)), \
+                        (nprocs, tsrc, tdst, rank, m)
                     # unpack the same buffer back along in_axis into a zeroed array
-                    td = torch.zeros_like(ts)
-                    dfft.unpack(tb, td, list(m), in_axis)
-                    refu = PU.unpack(ref, src.shape, list(m), in_axis)
-                    assert np.array_equal(td.cpu().numpy().view(np.uint8), refu.view(np.uint8))
+                    td = _filled(src.shape, 0, ts.dtype, dev, layout)
+                    dfft.unpack(tb, td, list(m), in_axis, layout=lay)
+                    refu = PU.unpack(ref, src.shape, list(m), in_axis, layout=layout)
+                    assert np.array_equal(np.ascontiguousarray(td.cpu().numpy()).view(np.uint8),
+                                          np.ascontiguousarray(refu).view(np.uint8))
+
+
+def test_pack_rejects_a_buffer_of_the_other_layout(dev):
+    import torch
+    import dfft
+    src = torch.zeros(8, 7, 5, dtype=torch.float64, device=dev)
+    right = torch.zeros(2, 8, 4, 5, dtype=torch.float64, device=dev)
+    with pytest.raises(dfft.DFFTError):
+        dfft.pack(src, right, [0, 1, 2], 1, layout=dfft.LayoutLeft)  # row-major arrays, LayoutLeft requested
 
 
 def test_all2all_closed_form(dev):

@@ -137,6 +137,45 @@ def test_layout_left_topologies(dev):
     _check(dev, [8, 7, 6], [0, 1, 2], [2, 3, 1], [1, 2, 3], True, tin_layout="left", tout_layout="right")
 
 
+@pytest.mark.parametrize("exec_mode", [0, 1])
+@pytest.mark.parametrize("fp64", [True, False])
+def test_layout_left_views(dev, exec_mode, fp64):
+    """LayoutLeft (column-major) views, the {float, double} x LayoutLeft cases
+    of Test_SlabPlan.cpp:13-16 / Test_PencilPlan.cpp: fused execution and the
+    block walk (LayoutLeft block analyses: maps with the FFT axes first, (e..., P)
+    buffers), slab and pencil, C2C and R2C, against the oracle."""
+    cases = [
+        ([8, 7, 5], [2, 1, 1], [1, 2, 1], [0, 1, 2]),
+        ([8, 7, 5], [1, 1, 3], [3, 1, 1], [2, 0, 1]),
+        ([9, 8, 6], [1, 2, 1], [2, 1, 1], [1, 2]),
+        ([8, 7, 6], [2, 2, 1], [1, 2, 2], [0, 1, 2]),
+        ([8, 7, 6], [1, 2, 3], [2, 3, 1], [0, 2, 1]),
+        ([3, 8, 7, 6], [1, 2, 1, 1], [1, 1, 2, 1], [1, 2, 3]),
+    ]
+    for gin, tin, tout, axes in cases:
+        for real in (False, True):
+            plan = _check(dev, gin, axes, tin, tout, real, fp64, exec_mode=exec_mode, seed=13, view_layout="left")
+            assert plan.name() in ("SlabPlan", "PencilPlan")
+
+
+def test_layout_left_views_with_left_topologies(dev):
+    _check(dev, [8, 7, 6], [0, 1, 2], [1, 2, 3], [3, 2, 1], True, tin_layout="right", tout_layout="left",
+           view_layout="left")
+    _check(dev, [8, 7, 6], [0, 1, 2], [2, 3, 1], [1, 2, 3], False, tin_layout="left", tout_layout="right",
+           view_layout="left", exec_mode=1)
+
+
+def test_layout_left_views_refuse_row_major_arrays(dev):
+    import torch
+    import dfft
+    ins = [torch.zeros(4, 8, 5, dtype=torch.complex128, device=dev) for _ in range(2)]
+    outs = [torch.zeros(8, 4, 5, dtype=torch.complex128, device=dev).permute(2, 1, 0).contiguous().permute(2, 1, 0)
+            for _ in range(2)]
+    plan = dfft.Plan(ins, outs, [0, 1, 2], [2, 1, 1], [1, 2, 1], layout=dfft.LayoutLeft)
+    with pytest.raises(dfft.DFFTError):
+        dfft.execute(plan, ins, outs, dfft.Direction.forward)  # inputs are row-major
+
+
 def test_chunked_and_tuned_launch_shapes(dev):
     """Launch-shape options, chunked first passes and the autotuner change the
     schedule, never the result."""

@@ -171,8 +171,11 @@ def _test_pack_expected(gu, tsrc_axis_len, nprocs, rank, split, src_local, buffe
     return ref
 
 
+@pytest.mark.parametrize("layout", [PL.LAYOUT_RIGHT, PL.LAYOUT_LEFT])
 @pytest.mark.parametrize("nprocs", [1, 2, 3, 4])
-def test_pack_matches_reference_construction(nprocs):
+def test_pack_matches_reference_construction(nprocs, layout):
+    """Test_Pack.cpp:386-473 for both buffer layouts: LayoutLeft buffers are
+    (e0, e1, e2, P) with the process index last (p = i3)."""
     gn = (8, 7, 5)
     i0, i1, i2 = np.meshgrid(*[np.arange(n) for n in gn], indexing="ij")
     gu = (i2 + i1 * gn[2] + i0 * gn[2] * gn[1]).astype(np.float64)
@@ -181,18 +184,36 @@ def test_pack_matches_reference_construction(nprocs):
         if nprocs == 1:
             continue
         split, _ = PL.slab_in_out_axes(tsrc, tdst)
-        bext = PL.compute_buffer_extents(PL.LAYOUT_RIGHT, list(gn), tsrc, tdst)
+        bext = PL.compute_buffer_extents(layout, list(gn), tsrc, tdst)
+        bright = PL.compute_buffer_extents(PL.LAYOUT_RIGHT, list(gn), tsrc, tdst)
+        assert bext == (bright if layout == PL.LAYOUT_RIGHT else bright[1:] + bright[:1])
         for rank in range(nprocs):
             loc = PE.scatter(gu, list(gn), tsrc)[rank]
-            exp = _test_pack_expected(gu, None, nprocs, rank, split, loc, bext)
+            exp = _test_pack_expected(gu, None, nprocs, rank, split, loc, bright)
+            if layout == PL.LAYOUT_LEFT:
+                exp = np.moveaxis(exp, 0, -1)
             for m in itertools.permutations(range(3)):
                 src = np.ascontiguousarray(loc.transpose(m))
-                got = PU.pack(src, bext, list(m), split)
+                got = PU.pack(src, bext, list(m), split, layout)
                 assert np.array_equal(got, exp), (tsrc, tdst, rank, m)
-                assert np.array_equal(PU.pack_literal(src, bext, list(m), split), exp)
-                back = PU.unpack(got, src.shape, list(m), split)
+                assert np.array_equal(PU.pack_literal(src, bext, list(m), split, layout), exp)
+                back = PU.unpack(got, src.shape, list(m), split, layout=layout)
                 assert np.array_equal(back, src)
-                assert np.array_equal(PU.unpack_literal(got, src.shape, list(m), split), src)
+                assert np.array_equal(PU.unpack_literal(got, src.shape, list(m), split, layout), src)
+
+
+def test_all2all_left_moves_the_last_axis():
+    """LayoutLeft buffers exchange blocks of their last (process) axis."""
+    sends = [np.arange(12.0).reshape(2, 2, 3) + 100 * r for r in range(3)]
+    recvs = PU.all2all(sends, PL.LAYOUT_LEFT)
+    right = PU.all2all([np.moveaxis(a, -1, 0) for a in sends])
+    for a, b in zip(recvs, right):
+        assert np.array_equal(a, np.moveaxis(b, 0, -1))
+    for q in range(3):
+        for p in range(3):
+            assert np.array_equal(recvs[q][..., p], sends[p][..., q])
+    with pytest.raises(RuntimeError):
+        PU.all2all([np.zeros((2, 3)), np.zeros((2, 3))], PL.LAYOUT_LEFT)
 
 
 def test_all2all_rejects_wrong_proc_extent():
