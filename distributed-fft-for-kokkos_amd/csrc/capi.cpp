@@ -406,6 +406,13 @@ int dfft_group_execute(dfft_group* group, void* const* in, void* const* out, int
   });
 }
 
+int dfft_group_execute_noalloc(dfft_group* group, void* const* in, void* const* out, int direction, void* stream) {
+  return guarded([&] {
+    need(group, "null group");
+    group->impl->execute_static(in, out, direction, static_cast<hipStream_t>(stream));
+  });
+}
+
 int dfft_group_num_launches(const dfft_group* group, int direction, int* nlaunches) {
   return guarded([&] {
     need(group && nlaunches, "null pointer");

@@ -207,19 +207,17 @@ std::vector<LinePass> Group::build_pass(const std::vector<void*>& in, const std:
   return passes;
 }
 
-void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, int dir, hipStream_t s) {
-  DirState& ds = dir_[dir];
-  check(hipStreamSynchronize(s), "hipStreamSynchronize");
+void Group::build_tables(const std::vector<void*>& in, const std::vector<void*>& out, int dir, Tables& t) const {
   const std::size_t np = static_cast<std::size_t>(max_passes_);
   const std::size_t ns = 2 * np;  // slots: (pass, one or two lines per lane)
-  ds.items.resize(ns, nullptr);
-  ds.first.resize(ns, nullptr);
-  ds.count.assign(ns, 0);
-  ds.total.assign(ns, 0);
-  ds.grid.assign(ns, 0);
-  ds.nt.assign(ns, 0);
-  ds.lds.assign(ns, 0);
-  ds.single.assign(np, {});
+  t.items.assign(ns, {});
+  t.first.assign(ns, {});
+  t.count.assign(ns, 0);
+  t.total.assign(ns, 0);
+  t.grid.assign(ns, 0);
+  t.nt.assign(ns, 0);
+  t.lds.assign(ns, 0);
+  t.single.assign(np, {});
   for (std::size_t k = 0; k < np; ++k) {
     std::vector<LinePass> all = build_pass(in, out, dir == 0, static_cast<int>(k));
     double bytes = 0.0;  // array bytes the pass streams through, all members
@@ -250,48 +248,68 @@ void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, 
         q.lines_per_block = 0;  // launcher heuristic
         q.values_per_thread = 0;
         q.dual_lines = 0;
-        ds.single[k].push_back(q);
+        t.single[k].push_back(q);
       }
     }
     for (int v = 0; v < 2; ++v) {
       const std::size_t slot = 2 * k + static_cast<std::size_t>(v);
-      ds.nt[slot] = nt;
-      ds.count[slot] = static_cast<int>(grouped[v].size());
-      ds.total[slot] = first[v].back();
-      ds.lds[slot] = lds[v];
+      t.nt[slot] = nt;
+      t.count[slot] = static_cast<int>(grouped[v].size());
+      t.total[slot] = first[v].back();
+      t.lds[slot] = lds[v];
       if (grouped[v].empty()) continue;
       const int resident = group_grid(lds[v], fp64_, v == 1);
       if (resident <= 0) throw Error(DFFT_ERROR_HIP, "occupancy query of the grouped kernel failed");
-      ds.grid[slot] = grid_mult_ > 0 ? resident * grid_mult_ : first[v].back();
-      if (!ds.items[slot]) {
-        check(hipMalloc(&ds.items[slot], sizeof(LinePass) * members_.size()), "hipMalloc");
-        check(hipMalloc(&ds.first[slot], sizeof(int32_t) * (members_.size() + 1)), "hipMalloc");
-      }
-      check(hipMemcpy(ds.items[slot], grouped[v].data(), sizeof(LinePass) * grouped[v].size(),
-                      hipMemcpyHostToDevice),
-            "hipMemcpy");
-      check(hipMemcpy(ds.first[slot], first[v].data(), sizeof(int32_t) * first[v].size(), hipMemcpyHostToDevice),
-            "hipMemcpy");
+      t.grid[slot] = grid_mult_ > 0 ? resident * grid_mult_ : first[v].back();
+      t.items[slot] = std::move(grouped[v]);
+      t.first[slot] = std::move(first[v]);
     }
+  }
+}
+
+void Group::upload(const std::vector<void*>& in, const std::vector<void*>& out, int dir, hipStream_t s) {
+  DirState& ds = dir_[dir];
+  check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  build_tables(in, out, dir, ds.tables);
+  const std::size_t ns = ds.tables.items.size();
+  ds.items.resize(ns, nullptr);
+  ds.first.resize(ns, nullptr);
+  for (std::size_t slot = 0; slot < ns; ++slot) {
+    if (ds.tables.count[slot] == 0) continue;
+    if (!ds.items[slot]) {
+      check(hipMalloc(&ds.items[slot], sizeof(LinePass) * members_.size()), "hipMalloc");
+      check(hipMalloc(&ds.first[slot], sizeof(int32_t) * (members_.size() + 1)), "hipMalloc");
+    }
+    check(hipMemcpy(ds.items[slot], ds.tables.items[slot].data(), sizeof(LinePass) * ds.tables.items[slot].size(),
+                    hipMemcpyHostToDevice),
+          "hipMemcpy");
+    check(hipMemcpy(ds.first[slot], ds.tables.first[slot].data(), sizeof(int32_t) * ds.tables.first[slot].size(),
+                    hipMemcpyHostToDevice),
+          "hipMemcpy");
   }
   ds.in = in;
   ds.out = out;
   ds.built = true;
 }
 
-void Group::launch_all(int dir, hipStream_t s) {
+void Group::launch_tables(const Tables& t, LinePass* const* items, int32_t* const* first, int dir,
+                          hipStream_t s) const {
   TraceRange range("dfft::Group %s (%zu members)", dir == 0 ? "forward" : "backward", members_.size());
-  const DirState& ds = dir_[dir];
   for (int k = 0; k < max_passes_; ++k) {
     for (int v = 0; v < 2; ++v) {
       const int slot = 2 * k + v;
-      if (ds.count[slot] > 0)
-        check(launch_group_pass(ds.items[slot], ds.first[slot], ds.count[slot], ds.total[slot], ds.grid[slot],
-                                ds.lds[slot], fp64_, ds.nt[slot] != 0, v == 1, s),
+      if (t.count[slot] > 0)
+        check(launch_group_pass(items[slot], first[slot], t.count[slot], t.total[slot], t.grid[slot], t.lds[slot],
+                                fp64_, t.nt[slot] != 0, v == 1, s),
               "grouped line pass");
     }
-    for (const LinePass& p : ds.single[k]) check(launch_line_pass(p, fp64_, s), "line pass");
+    for (const LinePass& p : t.single[k]) check(launch_line_pass(p, fp64_, s), "line pass");
   }
+}
+
+void Group::launch_all(int dir, hipStream_t s) {
+  const DirState& ds = dir_[dir];
+  launch_tables(ds.tables, ds.items.data(), ds.first.data(), dir, s);
 }
 
 int Group::num_launches(int direction) const {
@@ -299,8 +317,8 @@ int Group::num_launches(int direction) const {
   if (ds.built) {
     // the launches of the tables in use: grouped slots plus single passes
     int n = 0;
-    for (int c : ds.count) n += c > 0 ? 1 : 0;
-    for (const auto& v : ds.single) n += static_cast<int>(v.size());
+    for (int c : ds.tables.count) n += c > 0 ? 1 : 0;
+    for (const auto& v : ds.tables.single) n += static_cast<int>(v.size());
     return n;
   }
   // before the first execute: one grouped launch per pass (two where some of
@@ -419,31 +437,40 @@ void Group::autotune(void* const* in, void* const* out, hipStream_t s) {
   set_option("dual", best_dual);
 }
 
+int Group::check_call(const std::vector<void*>& vin, const std::vector<void*>& vout, int direction) const {
+  if (direction != DFFT_FORWARD && direction != DFFT_BACKWARD)
+    throw Error(DFFT_ERROR_INVALID_ARGUMENT, "Invalid FFT direction specified.");
+  const int dir = direction == DFFT_FORWARD ? 0 : 1;
+  for (std::size_t i = 0; i < members_.size(); ++i) {
+    if (!vin[i] || !vout[i]) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null member array");
+    if (members_[i].real && vin[i] == vout[i])
+      throw Error(DFFT_ERROR_ALIAS, "an R2C member cannot run in place");
+  }
+  // every member's allocations must hold its arrays
+  const std::size_t cs = fp64_ ? 16 : 8;
+  for (std::size_t i = 0; i < members_.size(); ++i) {
+    const Member& m = members_[i];
+    const std::size_t nreal = volume(m.gin) * (m.real ? cs / 2 : cs), ncplx = volume(m.gout) * cs;
+    const std::string who = "group member " + std::to_string(i);
+    check_device_extent(vin[i], dir == 0 ? nreal : ncplx, who + " input");
+    check_device_extent(vout[i], dir == 0 ? ncplx : nreal, who + " output");
+  }
+  return dir;
+}
+
 void Group::execute(void* const* in, void* const* out, int direction, bool use_graph, hipStream_t s) {
   if (!in || !out) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null pointer table");
   if (direction != DFFT_FORWARD && direction != DFFT_BACKWARD)
     throw Error(DFFT_ERROR_INVALID_ARGUMENT, "Invalid FFT direction specified.");
   const int dir = direction == DFFT_FORWARD ? 0 : 1;
   std::vector<void*> vin(in, in + members_.size()), vout(out, out + members_.size());
-  for (std::size_t i = 0; i < members_.size(); ++i) {
-    if (!vin[i] || !vout[i]) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null member array");
-    if (members_[i].real && vin[i] == vout[i])
-      throw Error(DFFT_ERROR_ALIAS, "an R2C member cannot run in place");
-  }
   int cur = -1;
   check(hipGetDevice(&cur), "hipGetDevice");
   if (cur != device_) check(hipSetDevice(device_), "hipSetDevice");
   DirState& ds = dir_[dir];
   if (!ds.built || ds.in != vin || ds.out != vout) {
-    // a new pointer set: every member's allocations must hold its arrays
-    const std::size_t cs = fp64_ ? 16 : 8;
-    for (std::size_t i = 0; i < members_.size(); ++i) {
-      const Member& m = members_[i];
-      const std::size_t nreal = volume(m.gin) * (m.real ? cs / 2 : cs), ncplx = volume(m.gout) * cs;
-      const std::string who = "group member " + std::to_string(i);
-      check_device_extent(vin[i], dir == 0 ? nreal : ncplx, who + " input");
-      check_device_extent(vout[i], dir == 0 ? ncplx : nreal, who + " output");
-    }
+    // a new pointer set: checked once, then the tables are rebuilt
+    check_call(vin, vout, direction);
     if (ds.exec) {
       check(hipStreamSynchronize(graph_stream_), "hipStreamSynchronize");
       (void)hipGraphExecDestroy(ds.exec);

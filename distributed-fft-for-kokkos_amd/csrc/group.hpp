@@ -26,6 +26,12 @@ class Group {
 
   // in[i] / out[i]: member i's arrays (inputs are not modified)
   void execute(void* const* in, void* const* out, int direction, bool use_graph, hipStream_t s);
+  // execute() without any device allocation: the pass tables go into a
+  // static device array of kStaticTableBytes (group_static.hip) shared by
+  // every group on the device; no graph.  Throws DFFT_ERROR_UNSUPPORTED when
+  // this call's tables do not fit.
+  void execute_static(void* const* in, void* const* out, int direction, hipStream_t s);
+  static constexpr std::size_t kStaticTableBytes = std::size_t(4) << 20;
   int count() const { return static_cast<int>(members_.size()); }
   // launches one execute of `direction` issues
   int num_launches(int direction) const;
@@ -50,8 +56,24 @@ class Group {
   };
   std::vector<LinePass> build_pass(const std::vector<void*>& in, const std::vector<void*>& out, bool forward,
                                    int k) const;
+  // host form of one direction's launches: slot 2k + v holds the grouped
+  // members of pass k taken one (v = 0) or two (v = 1) lines per lane
+  struct Tables {
+    std::vector<std::vector<LinePass>> items;
+    std::vector<std::vector<int32_t>> first;  // virtual-block prefix per slot
+    std::vector<int> count, total, grid, nt;
+    std::vector<size_t> lds;
+    std::vector<std::vector<LinePass>> single;  // per pass: individually launched passes
+  };
+  void build_tables(const std::vector<void*>& in, const std::vector<void*>& out, int dir, Tables& t) const;
+  // argument checks of execute(): pointers, direction, R2C aliasing, and that
+  // every member allocation holds its arrays; returns the direction index
+  int check_call(const std::vector<void*>& vin, const std::vector<void*>& vout, int direction) const;
   void upload(const std::vector<void*>& in, const std::vector<void*>& out, int dir, hipStream_t s);
   void launch_all(int dir, hipStream_t s);
+  // launches of one direction from tables whose slots sit at items[slot] /
+  // first[slot] on the device
+  void launch_tables(const Tables& t, LinePass* const* items, int32_t* const* first, int dir, hipStream_t s) const;
   void invalidate();
 
   std::vector<Member> members_;
@@ -69,13 +91,9 @@ class Group {
   struct DirState {
     std::vector<void*> in, out;
     bool built = false;
-    // grouped launches: slot 2k + v holds the members of pass k taken one
-    // line per lane (v = 0) or two lines per lane (v = 1)
-    std::vector<LinePass*> items;     // device, one array per slot
-    std::vector<int32_t*> first;      // device
-    std::vector<int> count, total, grid, nt;
-    std::vector<size_t> lds;
-    std::vector<std::vector<LinePass>> single;  // host, per pass: individually launched passes
+    Tables tables;                    // host form of the launches
+    std::vector<LinePass*> items;     // device copies of tables.items, one array per slot
+    std::vector<int32_t*> first;      // device copies of tables.first
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } dir_[2];

@@ -163,7 +163,8 @@ EXPORTED = [
     "dfft_unpack", "dfft_all2all", "dfft_exchange", "dfft_get_all_slab_topologies", "dfft_rank_to_coord",
     "dfft_compute_local_extents_and_starts",
     "dfft_get_all_pencil_topologies", "dfft_block_analyses", "dfft_group_create", "dfft_group_destroy",
-    "dfft_group_execute", "dfft_group_num_launches", "dfft_group_set_option", "dfft_group_get_option",
+    "dfft_group_execute", "dfft_group_execute_noalloc", "dfft_group_num_launches", "dfft_group_set_option",
+    "dfft_group_get_option",
     "dfft_group_autotune", "dfft_ns_create", "dfft_ns_destroy",
     "dfft_ns_local_extents", "dfft_ns_init_taylor_green", "dfft_ns_set_physical", "dfft_ns_set_state",
     "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_autotune", "dfft_ns_energy", "dfft_ns_velocity",
@@ -227,6 +228,7 @@ def load_library(path: os.PathLike | None = None):
         "dfft_group_destroy": ([ctypes.c_void_p], ctypes.c_int),
         "dfft_group_execute": ([ctypes.c_void_p, vpp, vpp, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
                                ctypes.c_int),
+        "dfft_group_execute_noalloc": ([ctypes.c_void_p, vpp, vpp, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
         "dfft_group_num_launches": ([ctypes.c_void_p, ctypes.c_int, ip], ctypes.c_int),
         "dfft_group_set_option": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long], ctypes.c_int),
         "dfft_group_get_option": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
@@ -624,6 +626,14 @@ class Group:
         self.check_views(ins, outs, direction)
         _check(self._lib.dfft_group_execute(self._h, _ptrs(ins), _ptrs(outs), int(direction), int(bool(use_graph)),
                                             _stream_handle(stream)))
+
+    def execute_noalloc(self, ins, outs, direction, stream=None):
+        """``execute`` without device allocation: the pass tables go into the
+        library's static table area (4 MiB per GPU, shared by every call on
+        the device; overlapping calls race)."""
+        self.check_views(ins, outs, direction)
+        _check(self._lib.dfft_group_execute_noalloc(self._h, _ptrs(ins), _ptrs(outs), int(direction),
+                                                    _stream_handle(stream)))
 
     def num_launches(self, direction):
         v = ctypes.c_int()

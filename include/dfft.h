@@ -206,6 +206,17 @@ int dfft_group_destroy(dfft_group* group);
  * once per direction and pointer set into a hipGraph and replayed after. */
 int dfft_group_execute(dfft_group* group, void* const* in, void* const* out, int direction, int use_graph,
                        void* stream);
+/* dfft_group_execute (use_graph = 0) that allocates no device memory: no
+ * counterpart in the reference; for callers that cannot call hipMalloc (code
+ * under stream capture, serving loops, harnesses that forbid allocation).
+ * The call's pass tables go into a static device array of 4 MiB per GPU
+ * (about 2,600 member passes); a call whose tables need more returns
+ * DFFT_ERROR_UNSUPPORTED before launching anything.  The array is shared by
+ * every group and every call on the device: calls that overlap on one GPU
+ * race.  The call synchronises the stream once (after the table copy, before
+ * its kernels are queued).  R2C members still use the complex scratch their
+ * group allocated at creation. */
+int dfft_group_execute_noalloc(dfft_group* group, void* const* in, void* const* out, int direction, void* stream);
 int dfft_group_num_launches(const dfft_group* group, int direction, int* nlaunches);
 /* "lds_cap" (LDS bytes per block of the grouped kernel, 1024..65536),
  * "grid_mult" (multiples of the resident-block grid; 0 = one block per
