@@ -58,6 +58,12 @@ struct LinePass {
   // adjacent elements of one row): two lines per lane, one 16-byte access
   // per element pair; ignored where the lines are not adjacent
   int32_t dual_lines;
+  // two-lines-per-lane passes only: bit 0 set when the input side's line
+  // pairs are loaded as two 8-byte elements instead of one aligned 16-byte
+  // pair -- an input with odd rows under a padded enumeration, whose pair at
+  // a row's end has its second line in the padding (that element is then not
+  // loaded)
+  int32_t dual_split;
   // Weight of the interior half-spectrum entries (0 < k < n/2) of an R2C /
   // C2R line, applied to the values a C2R line loads or an R2C line stores;
   // 0 means 1.  Adjoint transforms use 1/2 (C2R) and 2 (R2C).
@@ -87,7 +93,7 @@ bool group_line_shape(int n, bool fp64, size_t lds_cap, int* lines_per_block, si
 bool group_dual_shape(int n, size_t lds_cap, int* lines_per_block, size_t* lds_bytes);
 // true if two adjacent lines per lane apply to pass p with that many lines per
 // block (FP32 C2C, lanes across lines, consecutive lines adjacent row
-// elements on both sides, 16-byte aligned rank buffers)
+// elements on both sides, 16-byte aligned rank buffers; grouped passes)
 bool dual_lines_apply(const LinePass& p, int lines_per_block);
 // grid of the persistent grouped launch: resident blocks of the whole device
 int group_grid(size_t lds_bytes, bool fp64, bool dual);

@@ -588,7 +588,11 @@ template <typename Real, int N, int RM, int S> struct StageLoop2 {
   }
 };
 
-template <typename Real, int N, int RM, bool NT>
+// SPLIT_IN: the input side has odd rows under a padded enumeration; its
+// line pairs are loaded as two 8-byte elements, and the second element of a
+// pair whose second line is padding (the next row's first element there, or
+// past the end of the rank's array) is not loaded.
+template <typename Real, int N, int RM, bool NT, bool SPLIT_IN = false>
 __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
   using C = cplx_t<Real>;
   using Cfg = FastCfg<Real, N, RM>;
@@ -609,6 +613,8 @@ __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, 
   LineIO<Real, NT> io;
   if (valid) io = make_line_io<Real, NT>(p, la);
 
+  bool second = false;
+  if constexpr (SPLIT_IN) second = valid && !padding_line<Real>(p, la + 1);
   C va[R], vb[R];
   constexpr int r0 = Cfg::P.rad[0];
   constexpr int NB0 = R / r0;
@@ -618,8 +624,15 @@ __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, 
     const int64_t step = static_cast<int64_t>(N / r0) * io.is;
 #pragma unroll
     for (int j = 0; j < r0; ++j) {
-      const V4 x = valid ? gload<NT>(reinterpret_cast<const V4*>(src)) : V4{Real(0), Real(0), Real(0), Real(0)};
-      const C a{x.x, x.y}, c{x.z, x.w};
+      C a, c;
+      if constexpr (SPLIT_IN) {
+        a = valid ? gload_c<NT, Real>(src) : C{Real(0), Real(0)};
+        c = second ? gload_c<NT, Real>(src + 1) : C{Real(0), Real(0)};
+      } else {
+        const V4 x = valid ? gload<NT>(reinterpret_cast<const V4*>(src)) : V4{Real(0), Real(0), Real(0), Real(0)};
+        a = C{x.x, x.y};
+        c = C{x.z, x.w};
+      }
       va[q * r0 + j] = p.inverse ? cconj(a) : a;
       vb[q * r0 + j] = p.inverse ? cconj(c) : c;
       src += step;
@@ -1045,11 +1058,11 @@ __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   fast_line_body<Real, N, RM, NT, SPLIT>(p, blk, p.lines_per_block, smem_raw);
 }
 
-template <typename Real, int N, int RM, bool NT>
+template <typename Real, int N, int RM, bool NT, bool SPLIT_IN>
 __global__ void __launch_bounds__(1024) fft_line2_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
   const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
-  fast_line2_body<Real, N, RM, NT>(p, blk, p.lines_per_block, smem_raw);
+  fast_line2_body<Real, N, RM, NT, SPLIT_IN>(p, blk, p.lines_per_block, smem_raw);
 }
 
 // ---------------------------------------------------------------------------
@@ -1131,25 +1144,43 @@ template <typename Real, int N> constexpr bool dual_kernel_exists() {
   return std::is_same<Real, float>::value && N >= 128 && (N & (N - 1)) == 0;
 }
 
-inline bool dual_lines_apply_impl(const LinePass& p, int B) {
-  if (!p.dual_lines || p.kind != kC2C || !p.lines_fast || p.nbatch < 1 || B < 2 || B % 2) return false;
+// Returns -1 where two lines per lane do not apply, else the dual_split
+// mask of the pass.  An input side whose rows are odd is accepted when the
+// lines are enumerated over a padded extent of even pitch along the row axis
+// (pairs then never straddle two rows): its pairs are loaded as two 8-byte
+// elements and the padding half of a row's last pair is skipped.  (Storing
+// such a side as two 8-byte elements measured slower than the one-line
+// kernel on the 1024^3 x pass, so odd output rows keep one line per lane.)
+inline int dual_lines_mode(const LinePass& p, int B) {
+  if (!p.dual_lines || p.kind != kC2C || !p.lines_fast || p.nbatch < 1 || B < 2 || B % 2) return -1;
   const int j = p.nbatch - 1, fa = p.batch_axis[j];
   // an odd unpadded row length of a padded array is fine: the pair at the
   // end of a row then covers one padding element, which lies inside the row
   // pitch and is never read by any pass
-  if (p.batch_ext[j] % 2 || p.line_begin % 2 || p.nlines % 2) return false;
+  if (p.batch_ext[j] % 2 || p.line_begin % 2 || p.nlines % 2) return -1;
+  const bool padded_rows = p.pad_real > 0 && p.pad_axis == fa;
+  int mask = 0, bit = 1;
   for (const DDesc* d : {&p.in, &p.out}) {
     const int inner = d->layout_left ? d->map[0] : d->map[d->ndim - 1];
-    if (inner != fa || d->ntopo[fa] != 1) return false;
+    if (inner != fa || d->ntopo[fa] != 1) return -1;
     const int32_t row = (!d->layout_left && d->pitch > 0) ? d->pitch : d->ext[fa];
-    if (row % 2) return false;
     int ranks = 1;
     for (int a = 0; a < d->ndim; ++a) ranks *= d->ntopo[a];
+    bool aligned = true;
     for (int r = 0; r < ranks && r < kMaxRanks; ++r)
-      if (reinterpret_cast<uintptr_t>(d->ptr[r]) % 16 != 0) return false;
+      if (reinterpret_cast<uintptr_t>(d->ptr[r]) % 16 != 0) aligned = false;
+    if (row % 2) {
+      if (!padded_rows || bit != 1) return -1;
+      mask |= bit;
+    } else if (!aligned) {
+      return -1;
+    }
+    bit <<= 1;
   }
-  return true;
+  return mask;
 }
+
+inline bool dual_lines_apply_impl(const LinePass& p, int B) { return dual_lines_mode(p, B) == 0; }
 
 template <typename Real, int N, int RM>
 hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
@@ -1162,11 +1193,15 @@ hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
     const size_t smem = static_cast<size_t>(B) * line_lds_bytes<Real, N, RM, true>();
     if (P2 * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
     p.lines_per_block = B;
-    static std::atomic<size_t> lds_limit[2] = {64 * 1024, 64 * 1024};
+    static std::atomic<size_t> lds_limit[4] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
     const int nt = p.nontemporal ? 1 : 0;
-    const void* fn = nt ? reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true>)
-                        : reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false>);
-    const hipError_t e = raise_lds_limit(lds_limit[nt], fn, smem);
+    const bool split = (p.dual_split & 1) != 0;
+    const void* fn =
+        nt ? (split ? reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true, true>)
+                    : reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true, false>))
+           : (split ? reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false, true>)
+                    : reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false, false>));
+    const hipError_t e = raise_lds_limit(lds_limit[2 * (split ? 1 : 0) + nt], fn, smem);
     if (e != hipSuccess) return e;
     const unsigned grid = static_cast<unsigned>((static_cast<int64_t>(p.nlines) + B - 1) / B);
     if (grid == 0) return hipSuccess;
@@ -1176,9 +1211,11 @@ hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
       p.swizzle = (C > 1 && grid % static_cast<unsigned>(C) == 0 && grid / C > 1) ? C : 0;
     }
     if (nt) {
-      hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true>), dim3(grid), dim3(P2 * T), smem, s, p);
+      if (split) hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true, true>), dim3(grid), dim3(P2 * T), smem, s, p);
+      else hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true, false>), dim3(grid), dim3(P2 * T), smem, s, p);
     } else {
-      hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false>), dim3(grid), dim3(P2 * T), smem, s, p);
+      if (split) hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false, true>), dim3(grid), dim3(P2 * T), smem, s, p);
+      else hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false, false>), dim3(grid), dim3(P2 * T), smem, s, p);
     }
     return hipGetLastError();
   }
@@ -1192,7 +1229,11 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
   if constexpr (dual_kernel_exists<Real, N>()) {
     // default: 32 lines (256-byte row segments) or as many as 1024 lanes hold
     const int Bd = p.lines_per_block > 0 ? p.lines_per_block : (T <= 64 ? 32 : 2 * (1024 / T));
-    if (dual_lines_apply_impl(p, Bd)) return launch_dual<Real, N, RM>(p, Bd, s);
+    const int mode = dual_lines_mode(p, Bd);
+    if (mode >= 0) {
+      p.dual_split = mode;
+      return launch_dual<Real, N, RM>(p, Bd, s);
+    }
   }
   const bool split = split_available<Real, N>() && p.split_lds != 0;
   const size_t line_bytes = split ? line_lds_bytes<Real, N, RM, true>() : line_lds_bytes<Real, N, RM, false>();

@@ -176,6 +176,22 @@ def test_layout_left_views_refuse_row_major_arrays(dev):
         dfft.execute(plan, ins, outs, dfft.Direction.forward)  # inputs are row-major
 
 
+def test_fp32_two_lines_per_lane_from_odd_rows(dev):
+    """FP32 C2C passes take two adjacent lines per lane also when their input
+    has odd rows (the R2C spectrum's 257 elements per row, read under the
+    scratch's padded enumeration): the backward x pass here.  Same results
+    with two lines per lane on and off, against the oracle."""
+    gin, tin, tout = [256, 8, 512], [4, 1, 1], [1, 4, 1]
+
+    def off(plan, ins, outs):
+        for k in range(3):
+            plan.set_option(f"fwd.{k}.dual", 0)
+            plan.set_option(f"bwd.{k}.dual", 0)
+
+    for configure in (None, off):
+        _check(dev, gin, [0, 1, 2], tin, tout, True, fp64=False, seed=17, configure=configure)
+
+
 def test_chunked_and_tuned_launch_shapes(dev):
     """Launch-shape options, chunked first passes and the autotuner change the
     schedule, never the result."""
