@@ -29,8 +29,10 @@ Further sections of the same line:
   complex64, forward + backward, one grouped launch per pass, hipGraph;
 * precision: FP32 against FP64 (rel-L2) for 512^3 C2C and 1024^3 R2C on
   U[-1,1] inputs, and against the exact spectrum of sin(k.x), k = (3, 5, 7);
+* vendor_plan: rocFFT behind the plan boundary (exec mode EXEC_VENDOR, the
+  reference's TplPlan role) on the same views and round trip;
 * rocfft_comparator: the vendor single-GPU FFT (torch.fft -> hipFFT/rocFFT)
-  on the same 512^3 round trip, for reference;
+  on one global 512^3 array, for reference;
 * cpu_library_baseline: scipy.fft on all host cores beside the oracle.
 """
 
@@ -50,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 N = 512
 P = 4
 # measured with rocprofv3 --pmc on this workload (profiles/fft_line_fast_512cubed_c2c_slab4.md)
-TRAFFIC_BYTES_PER_LAUNCH = 4.2956e9
+TRAFFIC_BYTES_PER_LAUNCH = 4.2958e9
 TRAFFIC_SOURCE = "profiles/fft_line_fast_512cubed_c2c_slab4.md: FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per launch"
 
 
@@ -165,6 +167,29 @@ def cpu_library_baseline(warmup=3, runs=10):
             "kind": "library", "runs": runs, "warmup": warmup,
             "sample": f"scipy.fft fftn+ifftn of one 512^3 complex128 array, workers={cores}, "
                       f"median of {runs} after {warmup} warm-ups"}
+
+
+def vendor_plan(torch, dfft, dev, steps, warmup):
+    """rocFFT behind the plan boundary (dfft.EXEC_VENDOR, the reference's
+    TplPlan role): the same 512^3 slab round trip through the same views, the
+    ranks' blocks gathered into one array for one rocFFT call and scattered
+    back."""
+    out = {}
+    for tag, cdt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
+        ins, outs = make_inputs(torch, dfft, dev, cdt, seed=3)
+        backs = [torch.empty_like(a) for a in ins]
+        plan = dfft.Plan(ins, outs, [0, 1, 2], [P, 1, 1], [1, P, 1], exec_mode=dfft.EXEC_VENDOR)
+        ms = time_roundtrip(torch, dfft, plan, ins, outs, backs, steps, warmup)
+        fw, bw = pass_times(dfft, plan, ins, outs, backs, 2)
+        err = math.sqrt(sum(float(torch.linalg.vector_norm(b - a)) ** 2 for a, b in zip(ins, backs)) /
+                        sum(float(torch.linalg.vector_norm(a)) ** 2 for a in ins))
+        out[tag] = {"ms": round(ms, 4), "fwd_step_ms": [round(x, 4) for x in fw],
+                    "bwd_step_ms": [round(x, 4) for x in bw], "roundtrip_rel_l2": err}
+        del plan, ins, outs, backs
+        torch.cuda.empty_cache()
+    out["what"] = ("dfft plan with exec_mode=EXEC_VENDOR (plan name rocFFTPlan): gather remap, one rocFFT "
+                   "call, scatter remap per direction; steps = [gather, rocFFT, scatter]")
+    return out
 
 
 def rocfft_comparator(torch, dev, steps):
@@ -483,7 +508,10 @@ def main():
                 "avg_launch_ms": round(avg_launch, 4), "slowest_launch_ms": round(max(allp), 4),
                 # each launch's own read + write of the whole array over its duration
                 "pass_efficiency": round(pass_bytes / avg_launch / 1e6 / HBM_PEAK_GBS, 4),
-                "sweep_cap": round(1.0 / launches, 4)}
+                "sweep_cap": round(1.0 / launches, 4),
+                # measured HBM bytes of a launch over its compulsory share: one
+                # full sweep per axis pass (3.00 for 3 launches per transform)
+                "traffic_over_algorithmic": round(TRAFFIC_BYTES_PER_LAUNCH / alg_per_launch, 4)}
     line = {
         "metric": "3D FFT fwd+bwd ms & %HBM roofline @512^3 FP64 vs FP32; NS adjoint steps/s",
         "value": round(r64["ms"], 4),
@@ -504,7 +532,8 @@ def main():
     }
     run_sections(torch, dfft, dev, args, sections, line)
     if "rocfft" in sections:
-        log("[bench] rocFFT comparator ...")
+        log("[bench] rocFFT comparators ...")
+        line["vendor_plan"] = vendor_plan(torch, dfft, dev, max(3, args.steps // 2), 1)
         line["rocfft_comparator"] = rocfft_comparator(torch, dev, max(3, args.steps // 2))
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:

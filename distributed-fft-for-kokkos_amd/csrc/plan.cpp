@@ -136,7 +136,7 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
     if (!line_length_supported(static_cast<int>(n), fp64_))
       throw Error(DFFT_ERROR_UNSUPPORTED, "no line kernel for transform length " + std::to_string(n) +
                                               (fp64_ ? " in FP64" : " in FP32") +
-                                              " (lengths outside the fast set must fit one line in LDS)");
+                                              " (lengths outside the fast set must fit one line in LDS; DFFT_EXEC_VENDOR takes any length)");
   }
 
   check_hip(hipGetDevice(&device_), "hipGetDevice");
