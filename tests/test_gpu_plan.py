@@ -176,6 +176,21 @@ def test_layout_left_views_refuse_row_major_arrays(dev):
         dfft.execute(plan, ins, outs, dfft.Direction.forward)  # inputs are row-major
 
 
+@pytest.mark.parametrize("exec_mode", [0, 1, 2])
+def test_ranks_with_empty_blocks(dev, exec_mode):
+    """Extents smaller than a topology factor leave some ranks with empty
+    local views (compute_local_extents gives 0; MPI_Extents.hpp:195-235): the
+    other ranks' results are unchanged, empty views are never touched."""
+    for gin, tin, tout, real in [
+        ([3, 8, 8], [4, 1, 1], [1, 4, 1], False),
+        ([8, 3, 6], [4, 1, 1], [1, 4, 1], True),
+        ([2, 2, 9], [1, 1, 4], [4, 1, 1], False),
+        ([5, 6, 3], [2, 3, 1], [1, 2, 3], False),
+    ]:
+        for fp64 in (True, False):
+            _check(dev, gin, [0, 1, 2], tin, tout, real, fp64, exec_mode=exec_mode, seed=19)
+
+
 def test_fp32_two_lines_per_lane_from_odd_rows(dev):
     """FP32 C2C passes take two adjacent lines per lane also when their input
     has odd rows (the R2C spectrum's 257 elements per row, read under the
