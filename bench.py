@@ -226,15 +226,22 @@ def bench_ns(torch, steps):
     n = 256
     s = dfft_ns.NavierStokes(n, 2, 2, lx=2.0, nu=1.0 / 1600.0, dt=1e-3, form=dfft_ns.DIVERGENCE)
     s.autotune()
+    fused = s.get_option("products_fused")
     s.init_taylor_green(1.0)
     e0 = s.energy()
     s.step(2, use_graph=True)  # eager first step + capture
     ms = _sync_ms(torch, lambda: s.step(1, use_graph=True), steps)
     e1 = s.energy()
+    # the same step with the 6 product fields stored before the forward transform
+    s.set_option("fuse_products", 0)
+    s.init_taylor_green(1.0)
+    s.step(2, use_graph=True)
+    ms_stored = _sync_ms(torch, lambda: s.step(1, use_graph=True), steps)
     s.destroy()
     fields = 9 * 4  # transforms per step: (3 backward + 6 forward) x 4 stages
     return {"workload": "Taylor-Green 256^3 float64, 2x2 virtual ranks, RK4, divergence form, hipGraph",
             "ms_per_step": round(ms, 4), "steps_per_s": round(1e3 / ms, 3),
+            "products_in_forward_pass": fused, "stored_products_ms_per_step": round(ms_stored, 4),
             "transforms_per_step": fields, "energy_t0": e0, "energy_after": e1}
 
 

@@ -521,6 +521,20 @@ int dfft_ns_autotune(dfft_ns* ns, void* stream) {
   });
 }
 
+int dfft_ns_set_option(dfft_ns* ns, const char* key, long value) {
+  return guarded([&] {
+    need(ns && key, "null pointer");
+    ns->impl->set_option(key, value);
+  });
+}
+
+int dfft_ns_get_option(const dfft_ns* ns, const char* key, long* value) {
+  return guarded([&] {
+    need(ns && key && value, "null pointer");
+    *value = ns->impl->get_option(key);
+  });
+}
+
 int dfft_ns_energy(dfft_ns* ns, double* energy, void* stream) {
   return guarded([&] {
     need(ns && energy, "null pointer");

@@ -361,6 +361,15 @@ bool line_length_supported(int n, bool fp64) {
   return static_cast<size_t>(2) * static_cast<size_t>(n) * csize <= kGenericLdsMax;
 }
 
+bool products_pass_supported(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+
+hipError_t launch_products_pass(const LinePass& p, hipStream_t s) {
+  if (p.nlines <= 0) return hipSuccess;
+  if (!products_pass_supported(p.n) || p.in.ntopo[p.axis] != 1 || p.out.ntopo[p.axis] != 1)
+    return hipErrorNotSupported;
+  return dispatch_products_f64(p, s);
+}
+
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s) {
   if (p.nlines <= 0) return hipSuccess;
   // the compile-time kernel needs the line axis undistributed on both sides

@@ -64,6 +64,13 @@ struct LinePass {
   // a row's end has its second line in the padding (that element is then not
   // loaded)
   int32_t dual_split;
+  // FP64 R2C passes only (pair kernel): 1 = the line of field f (the batch
+  // axis prod_axis, six fields) is the pointwise product u_I * u_J of two of
+  // the three fields that the input descriptor holds along that axis, (I, J)
+  // = (0,0) (0,1) (0,2) (1,1) (1,2) (2,2) for f = 0..5 -- the products of the
+  // divergence-form Navier-Stokes right-hand side formed as the lines load
+  int32_t products;
+  int32_t prod_axis;
   // Weight of the interior half-spectrum entries (0 < k < n/2) of an R2C /
   // C2R line, applied to the values a C2R line loads or an R2C line stores;
   // 0 means 1.  Adjoint transforms use 1/2 (C2R) and 2 (R2C).
@@ -105,6 +112,10 @@ bool has_fast_line_kernel(int n);
 // kernel (one line of n complex values, twice, in at most 160 KB of LDS)
 bool line_length_supported(int n, bool fp64);
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
+// an FP64 R2C pass with p.products set (paired lines, lengths 8..1024 powers
+// of two); hipErrorNotSupported for other passes
+hipError_t launch_products_pass(const LinePass& p, hipStream_t s);
+bool products_pass_supported(int n);
 hipError_t launch_remap(const RemapArgs& a, int esize, hipStream_t s);
 
 // Reference-exact pack/unpack of one rank's local array (mapped by `map`)

@@ -224,7 +224,7 @@ template <typename Real, bool NT = false> struct LineIO {
 // extensions of both half spectra; the inverse transform of Z is a + i b.
 template <typename Real, bool NT = false> struct PairIO {
   using C = cplx_t<Real>;
-  const char* ib[2];
+  const char* ib[2];  // products passes: ib[0] = the velocity line this slot stages
   char* ob[2];
   int64_t is[2], os[2];  // per line: the two lines may lie in different rank blocks
   int32_t kind, n;
@@ -302,7 +302,27 @@ __device__ __forceinline__ LineIO<Real, NT> make_line_io(const LinePass& p, int3
   return io;
 }
 
-template <typename Real, bool NT>
+// factors of product field f (0..5) of the divergence-form products
+__device__ __forceinline__ int32_t prod_first(int32_t f) { return f < 3 ? 0 : (f < 5 ? 1 : 2); }
+__device__ __forceinline__ int32_t prod_second(int32_t f) { return f < 3 ? f : (f < 5 ? f - 2 : 2); }
+
+// Products pass: line `idx` is product field f = idx[prod_axis] of one
+// position; the velocity field (f / 2) of that position, which the slot of
+// the pair (f, f + 1) stages for the position's three slots.  Selects over the
+// unrolled axes keep idx in registers.
+__device__ __forceinline__ const char* product_source(const LinePass& p, const int32_t* idx, int64_t ies,
+                                                      int64_t& is) {
+  int32_t f = 0;
+#pragma unroll
+  for (int a = 0; a < kMaxDim; ++a)
+    if (a == p.prod_axis) f = idx[a];
+  int32_t j[kMaxDim];
+#pragma unroll
+  for (int a = 0; a < kMaxDim; ++a) j[a] = a == p.prod_axis ? f / 2 : idx[a];
+  return element_ptr(p.in, j, ies, p.axis, is);
+}
+
+template <typename Real, bool NT, bool PROD = false>
 __device__ __forceinline__ PairIO<Real, NT> make_pair_io(const LinePass& p, int32_t pair) {
   using C = cplx_t<Real>;
   PairIO<Real, NT> io;
@@ -311,12 +331,21 @@ __device__ __forceinline__ PairIO<Real, NT> make_pair_io(const LinePass& p, int3
   int32_t idx[kMaxDim];
   const int32_t la = 2 * pair;
   decode_line<Real>(p, la, idx);
-  io.ib[0] = element_ptr(p.in, idx, ies, p.axis, io.is[0]);
+  if constexpr (PROD) {
+    io.ib[0] = product_source(p, idx, ies, io.is[0]);
+  } else {
+    io.ib[0] = element_ptr(p.in, idx, ies, p.axis, io.is[0]);
+  }
   io.ob[0] = element_ptr(p.out, idx, oes, p.axis, io.os[0]);
   io.has_b = la + 1 < p.nlines;
   if (io.has_b) {
     decode_line<Real>(p, la + 1, idx);
-    io.ib[1] = element_ptr(p.in, idx, ies, p.axis, io.is[1]);
+    if constexpr (PROD) {
+      io.ib[1] = io.ib[0];
+      io.is[1] = io.is[0];
+    } else {
+      io.ib[1] = element_ptr(p.in, idx, ies, p.axis, io.is[1]);
+    }
     io.ob[1] = element_ptr(p.out, idx, oes, p.axis, io.os[1]);
   } else {
     io.ib[1] = io.ib[0];
@@ -827,7 +856,9 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
 
 // Paired R2C / C2R: slot b of the block transforms lines 2q and 2q+1
 // (q = blk*B + b) as one complex line; same stage code as fast_line_body.
-template <typename Real, int N, int RM, bool NT, bool SPLIT>
+// PROD: a products pass (LinePass::products): each loaded real value is the
+// product of the two factor lines' values at that position.
+template <typename Real, int N, int RM, bool NT, bool SPLIT, bool PROD = false>
 __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
   using C = cplx_t<Real>;
   using Cfg = FastCfg<Real, N, RM>;
@@ -852,7 +883,31 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
   void* buf = lds + b * line_lds_bytes<Real, N, RM, SPLIT>();
 
   PairIO<Real, NT> io{};
-  if (valid) io = make_pair_io<Real, NT>(p, pair);
+  if (valid) io = make_pair_io<Real, NT, PROD>(p, pair);
+
+  // Products pass: the block holds whole positions (B is a multiple of 3 and
+  // the field axis is enumerated fastest, so slots 3m..3m+2 are the pairs
+  // (0,1), (2,3), (4,5) of one position).  Slot 3m+q stages velocity line q of
+  // the position into the LDS of the position's three slots; each slot then
+  // forms its two product lines from LDS.  Global loads: 3 lines per
+  // position instead of 2 per product line.
+  [[maybe_unused]] const Real* stage = nullptr;
+  [[maybe_unused]] int pq = 0;
+  if constexpr (PROD) {
+    pq = b % 3;
+    Real* st = reinterpret_cast<Real*>(lds + (b - pq) * line_lds_bytes<Real, N, RM, SPLIT>());
+    if (valid) {
+      const Real* src = reinterpret_cast<const Real*>(io.ib[0]) + static_cast<int64_t>(t) * io.is[0];
+      const int64_t step = static_cast<int64_t>(T) * io.is[0];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        st[pq * N + t + j * T] = gload<NT>(src);
+        src += step;
+      }
+    }
+    __syncthreads();
+    stage = st;
+  }
 
   // line elements are walked with pointer increments (positions beta + j*S
   // on load, t + j*T on store): no 64-bit product per element
@@ -872,8 +927,16 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
         const Real* pb = reinterpret_cast<const Real*>(io.ib[1]) + static_cast<int64_t>(beta) * io.is[1];
 #pragma unroll
         for (int j = 0; j < r0; ++j) {
-          const Real xa = valid ? gload<NT>(pa) : Real(0);
-          const Real xb = vb ? gload<NT>(pb) : Real(0);
+          Real xa, xb;
+          if constexpr (PROD) {
+            const int pos = beta + j * S;
+            const int fa = 2 * pq, fb = 2 * pq + 1;
+            xa = valid ? stage[prod_first(fa) * N + pos] * stage[prod_second(fa) * N + pos] : Real(0);
+            xb = vb ? stage[prod_first(fb) * N + pos] * stage[prod_second(fb) * N + pos] : Real(0);
+          } else {
+            xa = valid ? gload<NT>(pa) : Real(0);
+            xb = vb ? gload<NT>(pb) : Real(0);
+          }
           v[q * r0 + j] = C{xa, xb};
           pa += sa;
           pb += sb;
@@ -953,6 +1016,7 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
       }
     }
   }
+  if constexpr (PROD) __syncthreads();  // the staged lines share LDS with the exchanges
   StageLoop<Real, N, RM, SPLIT, 0>::run(v, t, tw, buf);
   final_regroup<Real, N, RM, SPLIT>(v, t, buf);
   const int64_t ta = static_cast<int64_t>(T) * io.os[0], tb = static_cast<int64_t>(T) * io.os[1];
@@ -1044,11 +1108,11 @@ __device__ __forceinline__ int32_t pass_block(const LinePass& p, int32_t b, int3
   return b;
 }
 
-template <typename Real, int N, int RM, bool NT, bool SPLIT>
+template <typename Real, int N, int RM, bool NT, bool SPLIT, bool PROD = false>
 __global__ void __launch_bounds__(1024) fft_pair_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
   const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
-  fast_pair_body<Real, N, RM, NT, SPLIT>(p, blk, p.lines_per_block, smem_raw);
+  fast_pair_body<Real, N, RM, NT, SPLIT, PROD>(p, blk, p.lines_per_block, smem_raw);
 }
 
 template <typename Real, int N, int RM, bool NT, bool SPLIT>
@@ -1292,6 +1356,52 @@ hipError_t launch_fast_r(const LinePass& p, hipStream_t s) {
   }
 }
 
+// FP64 products pass: the paired R2C kernel (8 values per thread) with the
+// products formed at the loads; block shape as launch_fast's pair path.
+template <int N>
+hipError_t launch_products(LinePass p, hipStream_t s) {
+  using Real = double;
+  using Cfg = FastCfg<Real, N, 8>;
+  constexpr int T = Cfg::T;
+  if (!p.products || p.kind != kR2C || p.inverse) return hipErrorInvalidValue;
+  const bool split = p.split_lds != 0;
+  const size_t line_bytes = split ? line_lds_bytes<Real, N, 8, true>() : line_lds_bytes<Real, N, 8, false>();
+  // pairs per block: whole positions (3 pairs each), near the pass's own
+  // choice (tuned for its values per thread, which may differ from 8), cut
+  // to what this shape can hold
+  int B = p.lines_per_block;
+  if (B <= 0) B = p.lines_fast ? 8 : (T >= 256 ? 1 : 256 / T);
+  B = B >= 3 ? B / 3 * 3 : 3;
+  while (B > 3 && (B * T > 1024 || static_cast<size_t>(B) * line_bytes > 160 * 1024)) B -= 3;
+  const size_t smem = static_cast<size_t>(B) * line_bytes;
+  if (B * T > 1024 || smem > 160 * 1024 || p.nlines % 6 != 0 || p.line_begin % 6 != 0)
+    return hipErrorInvalidConfiguration;
+  // the staged velocity lines (3 per position) fit in the three slots' LDS
+  static_assert(3 * sizeof(Real) * N <= 3 * line_lds_bytes<Real, N, 8, true>(), "staging area");
+  p.lines_per_block = B;
+  p.swizzle = 0;
+  const bool nt = p.nontemporal != 0;
+  static std::atomic<size_t> lds_limit[4] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
+  const void* fn = split ? (nt ? reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, true, true, true>)
+                               : reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, false, true, true>))
+                         : (nt ? reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, true, false, true>)
+                               : reinterpret_cast<const void*>(&fft_pair_fast<Real, N, 8, false, false, true>));
+  const hipError_t e = raise_lds_limit(lds_limit[2 * (split ? 1 : 0) + (nt ? 1 : 0)], fn, smem);
+  if (e != hipSuccess) return e;
+  const int64_t units = (static_cast<int64_t>(p.nlines) + 1) / 2;
+  const unsigned grid = static_cast<unsigned>((units + B - 1) / B);
+  if (grid == 0) return hipSuccess;
+  const dim3 g(grid), b(static_cast<unsigned>(T * B));
+  if (split) {
+    if (nt) hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, true, true, true>), g, b, smem, s, p);
+    else hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, false, true, true>), g, b, smem, s, p);
+  } else {
+    if (nt) hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, true, false, true>), g, b, smem, s, p);
+    else hipLaunchKernelGGL((fft_pair_fast<Real, N, 8, false, false, true>), g, b, smem, s, p);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
 
 // fast-kernel dispatch for one precision and a range of lengths (defined in
@@ -1300,5 +1410,6 @@ hipError_t dispatch_fast_f64_short(const LinePass& p, hipStream_t s);
 hipError_t dispatch_fast_f64_long(const LinePass& p, hipStream_t s);
 hipError_t dispatch_fast_f32_short(const LinePass& p, hipStream_t s);
 hipError_t dispatch_fast_f32_long(const LinePass& p, hipStream_t s);
+hipError_t dispatch_products_f64(const LinePass& p, hipStream_t s);
 
 }  // namespace dfft

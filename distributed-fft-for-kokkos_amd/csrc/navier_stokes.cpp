@@ -128,13 +128,46 @@ std::vector<void*> NavierStokes::alloc_set(bool spectral, int nf) {
   return out;
 }
 
+void NavierStokes::forward_products(void* const* u, hipStream_t s) {
+  if (fuse_products_ && plan_f_->products_supported()) {
+    plan_f_->execute_forward_products(u, nl_.data(), s);
+    return;
+  }
+  for (int r = 0; r < P_; ++r)
+    hip_check(ns_products(static_cast<const double*>(u[r]), static_cast<double*>(pf_[r]), phys_pts(r), s),
+              "ns_products");
+  plan_f_->execute(pf_.data(), nl_.data(), DFFT_FORWARD, s);
+}
+
+void NavierStokes::set_option(const std::string& key, long value) {
+  if (key == "fuse_products") {
+    if (value != 0 && value != 1) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "fuse_products must be 0 or 1");
+    if (graph_exec_) {  // the captured step holds the old launches
+      hip_check(hipStreamSynchronize(graph_stream_), "hipStreamSynchronize");
+      (void)hipGraphExecDestroy(graph_exec_);
+      graph_exec_ = nullptr;
+    }
+    if (graph_) {
+      (void)hipGraphDestroy(graph_);
+      graph_ = nullptr;
+    }
+    fuse_products_ = value != 0;
+    return;
+  }
+  throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown NS option " + key);
+}
+
+long NavierStokes::get_option(const std::string& key) const {
+  if (key == "fuse_products") return fuse_products_ ? 1 : 0;
+  if (key == "products_fused") return (form_ == 0 && fuse_products_ && plan_f_->products_supported()) ? 1 : 0;
+  throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown NS option " + key);
+}
+
 void NavierStokes::rk_stage(int stage, hipStream_t s, void* const* u_keep) {
   if (form_ == 0) {
     void* const* u = u_keep ? u_keep : pb_.data();
     plan_b_->execute(y_.data(), u, DFFT_BACKWARD, s);
-    for (int r = 0; r < P_; ++r)
-      hip_check(ns_products(static_cast<const double*>(u[r]), static_cast<double*>(pf_[r]), phys_pts(r), s),
-                "ns_products");
+    forward_products(u, s);
   } else {
     for (int r = 0; r < P_; ++r)
       hip_check(ns_gradients(g_, sblk_[r], static_cast<double2*>(y_[r]), static_cast<double2*>(bin_[r]), s),
@@ -143,8 +176,8 @@ void NavierStokes::rk_stage(int stage, hipStream_t s, void* const* u_keep) {
     for (int r = 0; r < P_; ++r)
       hip_check(ns_advection(static_cast<double*>(pb_[r]), static_cast<double*>(pf_[r]), phys_pts(r), s),
                 "ns_advection");
+    plan_f_->execute(pf_.data(), nl_.data(), DFFT_FORWARD, s);
   }
-  plan_f_->execute(pf_.data(), nl_.data(), DFFT_FORWARD, s);
   for (int r = 0; r < P_; ++r) {
     NsStage st{form_, stage, static_cast<const double2*>(nl_[r]), static_cast<double2*>(y_[r]),
                static_cast<double2*>(yc_[r]), static_cast<double2*>(k1_[r]), static_cast<double2*>(k2_[r]),

@@ -31,6 +31,11 @@ class NavierStokes {
 
   void init_taylor_green(double v0, hipStream_t s);
   void autotune(hipStream_t s);  // tunes every plan on scratch buffers; keeps the state
+  // "fuse_products" (1, default): the divergence form's 6 products are formed
+  // in the forward transform's first pass from the 3 velocity fields where
+  // the plan allows it; 0: a separate products kernel writes the 6 fields
+  void set_option(const std::string& key, long value);
+  long get_option(const std::string& key) const;
   void step(int nsteps, bool use_graph, hipStream_t s);
   double energy(hipStream_t s);
   void get_state(void* const* dst, hipStream_t s) const;
@@ -80,6 +85,10 @@ class NavierStokes {
   std::vector<void*> alloc_set(bool spectral, int nf);
   void ensure_aux();
 
+  // forward transform of the 6 products of the velocity blocks u (fused into
+  // the R2C pass when enabled and supported, else products kernel + transform)
+  void forward_products(void* const* u, hipStream_t s);
+  bool fuse_products_ = true;
   int n_, px_, py_, P_, form_;
   NsGrid g_{};
   double lx_;

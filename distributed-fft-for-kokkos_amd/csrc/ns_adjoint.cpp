@@ -42,9 +42,7 @@ void NavierStokes::apply_rhs(void* const* y3, void* const* out3, hipStream_t s) 
     hip_check(ns_dealias(g_, sblk_[r], static_cast<const double2*>(y3[r]), static_cast<double2*>(ym[r]), s),
               "dealias");
   plan_b_->execute(ym.data(), pb_.data(), DFFT_BACKWARD, s);
-  for (int r = 0; r < P_; ++r)
-    hip_check(ns_products(static_cast<double*>(pb_[r]), static_cast<double*>(pf_[r]), phys_pts(r), s), "products");
-  plan_f_->execute(pf_.data(), nl_.data(), DFFT_FORWARD, s);
+  forward_products(pb_.data(), s);
   for (int r = 0; r < P_; ++r)
     hip_check(ns_rhs_only(g_, sblk_[r], static_cast<const double2*>(nl_[r]), static_cast<const double2*>(ym[r]),
                           static_cast<double2*>(out3[r]), s),

@@ -521,6 +521,77 @@ void Plan::run_fused(void* const* in, void* const* out, bool forward, hipStream_
   }
 }
 
+bool Plan::products_supported() const {
+  if (!fp64_ || !real_ || d_.exec_mode != DFFT_EXEC_FUSED || view_layout_ != Layout::Right) return false;
+  if (rank_ != 4 || gin_[0] != 6 || in_topo_[0] != 1 || out_topo_[0] != 1) return false;
+  if (fwd_.empty() || fwd_[0].kind != kR2C || fwd_[0].axis != 3 || fwd_[0].paired == 0) return false;
+  for (int a : {1, 2, 3})
+    if (std::find(axes_.begin(), axes_.end(), static_cast<std::size_t>(a)) == axes_.end()) return false;
+  return products_pass_supported(fwd_[0].n);
+}
+
+void Plan::execute_forward_products(void* const* u3, void* const* out, hipStream_t stream) {
+  if (!products_supported()) throw Error(DFFT_ERROR_UNSUPPORTED, "this plan cannot form the products in its R2C pass");
+  if (u3 == nullptr || out == nullptr) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null pointer table");
+  int cur = -1;
+  check_hip(hipGetDevice(&cur), "hipGetDevice");
+  if (cur != device_) check_hip(hipSetDevice(device_), "hipSetDevice");
+  if (total_size(gin_) == 0) return;
+  TraceRange range("dfft::%s::execute forward (products of 3 fields)", name_.c_str());
+  const auto& passes = fwd_;
+  const double scale = exec_scale(true);
+  // the 6-field input distribution (line enumeration) and the 3-field one
+  // the products are read from: the field axis is not distributed, so every
+  // rank's block of the 3-field array has the same strides but 3 fields
+  DDesc src6 = user_desc(gin_, in_topo_, in_tl_, u3);
+  DDesc src3 = src6;
+  src3.ext[0] = 3;
+  src3.base[0] = 3;
+  src3.rem[0] = 0;
+  src3.div_big[0] = FastDiv(4u);
+  src3.div_small[0] = FastDiv(3u);
+  DDesc dst = user_desc(gout_, out_topo_, out_tl_, out);
+  DDesc scr = scratch_desc(gout_, scratch_, 16);
+  for (std::size_t i = 0; i < passes.size(); ++i) {
+    const bool first = i == 0, last = i + 1 == passes.size();
+    DDesc b = last ? dst : scr;
+    if (first) {
+      LinePass p = make_pass(passes[0], src6, b, false, last ? scale : 1.0);
+      p.in = src3;
+      p.products = 1;
+      p.prod_axis = 0;
+      // enumerate the field axis fastest: the six product lines of one
+      // position sit in neighbouring pairs and read the same three velocity
+      // lines, which the caches serve after the first read
+      int fj = -1;
+      for (int j = 0; j < p.nbatch; ++j)
+        if (p.batch_axis[j] == 0) fj = j;
+      if (fj >= 0) {
+        const int32_t ax = p.batch_axis[fj], ex = p.batch_ext[fj];
+        const FastDiv dv = p.batch_div[fj];
+        for (int j = fj; j + 1 < p.nbatch; ++j) {
+          p.batch_axis[j] = p.batch_axis[j + 1];
+          p.batch_ext[j] = p.batch_ext[j + 1];
+          p.batch_div[j] = p.batch_div[j + 1];
+        }
+        p.batch_axis[p.nbatch - 1] = ax;
+        p.batch_ext[p.nbatch - 1] = ex;
+        p.batch_div[p.nbatch - 1] = dv;
+      }
+      TraceRange pr("dfft::forward pass 0 axis %d (products of the velocity lines)", passes[0].axis);
+      mark(stream);
+      check_hip(launch_products_pass(p, stream), "products pass");
+      mark(stream);
+      continue;
+    }
+    LinePass p = make_pass(passes[i], scr, b, false, last ? scale : 1.0);
+    TraceRange pr("dfft::forward pass %zu axis %d%s", i, passes[i].axis, last ? " (scatter to the output ranks)" : "");
+    mark(stream);
+    check_hip(launch_line_pass(p, fp64_, stream), "line pass");
+    mark(stream);
+  }
+}
+
 void Plan::execute(void* const* in, void* const* out, int direction, hipStream_t stream) {
   if (in == nullptr || out == nullptr) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "null pointer table");
   if (direction != DFFT_FORWARD && direction != DFFT_BACKWARD)

@@ -52,6 +52,16 @@ class Plan {
   // adjoint of execute(direction) under the real inner product Re sum conj(a) b
   // of each side: in has the shape of that direction's output, out of its input
   void execute_adjoint(void* const* in, void* const* out, int direction, hipStream_t stream);
+  // Forward execute of a 6-field R2C plan over 4-D views (field axis first,
+  // not distributed, transform axes 1..3) whose input fields are the products
+  // u_i u_j of the 3 velocity fields in `u3` (per rank: 3 fields shaped like
+  // the plan's input blocks), (i, j) = (0,0) (0,1) (0,2) (1,1) (1,2) (2,2):
+  // the products are formed as the first (R2C) pass loads its lines, so the
+  // 6 product fields are never stored.  products_supported() says whether
+  // this plan can (FP64, fused executor, a paired R2C first pass of a
+  // power-of-two length 8..1024).
+  bool products_supported() const;
+  void execute_forward_products(void* const* u3, void* const* out, hipStream_t stream);
   // execute with a HIP event pair around every launch; durations in ms
   std::vector<float> execute_timed(void* const* in, void* const* out, int direction, hipStream_t stream);
   int num_passes(int direction) const;

@@ -167,7 +167,7 @@ EXPORTED = [
     "dfft_group_get_option",
     "dfft_group_autotune", "dfft_ns_create", "dfft_ns_destroy",
     "dfft_ns_local_extents", "dfft_ns_init_taylor_green", "dfft_ns_set_physical", "dfft_ns_set_state",
-    "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_autotune", "dfft_ns_energy", "dfft_ns_velocity",
+    "dfft_ns_get_state", "dfft_ns_step", "dfft_ns_autotune", "dfft_ns_set_option", "dfft_ns_get_option", "dfft_ns_energy", "dfft_ns_velocity",
     "dfft_ns_vorticity",
     "dfft_ns_energy_spectrum", "dfft_ns_diag", "dfft_ns_rhs", "dfft_ns_rhs_tangent",
     "dfft_ns_rhs_adjoint", "dfft_ns_gradient",
@@ -242,6 +242,8 @@ def load_library(path: os.PathLike | None = None):
         "dfft_ns_get_state": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_step": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_autotune": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+        "dfft_ns_set_option": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_long], ctypes.c_int),
+        "dfft_ns_get_option": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "dfft_ns_energy": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_velocity": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),
         "dfft_ns_vorticity": ([ctypes.c_void_p, vpp, ctypes.c_void_p], ctypes.c_int),

@@ -89,6 +89,16 @@ class NavierStokes:
         """Tune the launch shapes of every plan of the solver (the state is kept)."""
         _check(self._lib.dfft_ns_autotune(self._h, _stream_handle(stream)))
 
+    def set_option(self, key, value):
+        """"fuse_products" (1 default / 0): form the 6 products inside the
+        forward transform's first pass instead of storing them."""
+        _check(self._lib.dfft_ns_set_option(self._h, key.encode(), int(value)))
+
+    def get_option(self, key):
+        v = ctypes.c_long()
+        _check(self._lib.dfft_ns_get_option(self._h, key.encode(), ctypes.byref(v)))
+        return int(v.value)
+
     def energy(self, stream=None) -> float:
         e = ctypes.c_double()
         _check(self._lib.dfft_ns_energy(self._h, ctypes.byref(e), _stream_handle(stream)))

@@ -260,6 +260,12 @@ int dfft_ns_step(dfft_ns* ns, int nsteps, int use_graph, void* stream);
  * buffers; the state is kept) and keep the fastest -- no counterpart in the
  * reference, whose plans are created once with vendor defaults */
 int dfft_ns_autotune(dfft_ns* ns, void* stream);
+/* "fuse_products" (1 default, 0): form the divergence form's 6 products in
+ * the forward transform's first pass from the velocity lines (no product
+ * fields stored) where the plan allows it; "products_fused" (read only): 1
+ * when the step runs that way.  No counterpart in the reference. */
+int dfft_ns_set_option(dfft_ns* ns, const char* key, long value);
+int dfft_ns_get_option(const dfft_ns* ns, const char* key, long* value);
 /* kinetic energy, FP64 reduction (diag_energy, main.cpp:855-910) */
 int dfft_ns_energy(dfft_ns* ns, double* energy, void* stream);
 /* diagnostics (diag_fields / diag_energy, main.cpp:812-935): physical

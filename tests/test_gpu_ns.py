@@ -86,6 +86,44 @@ def test_random_field_steps_match_oracle(dev, form):
     assert _rel(_gather(s, s.get_state(), 1), np.stack(y)) < 1e-12
 
 
+@pytest.mark.parametrize("n,px,py", [(16, 1, 1), (16, 2, 2), (32, 2, 1), (16, 1, 3), (64, 2, 2)])
+def test_products_formed_in_the_forward_pass_match_stored_products(dev, n, px, py):
+    """The divergence form's 6 products formed at the loads of the forward
+    R2C pass (no product fields in HBM) against the stored-products path."""
+    nu, dt = 0.01, 1e-2
+    rng = np.random.default_rng(n + 7 * px + 13 * py)
+    u = [rng.standard_normal((n, n, n)) for _ in range(3)]
+    a, b = _solver(n, px, py, 0, nu, dt), _solver(n, px, py, 0, nu, dt)
+    b.set_option("fuse_products", 0)
+    assert a.get_option("products_fused") == 1
+    assert b.get_option("products_fused") == 0 and b.get_option("fuse_products") == 0
+    for s in (a, b):
+        s.set_physical(_scatter(s, u, 0, dev))
+    ya = a.get_state()
+    assert _rel(_gather(a, a.rhs(ya), 1), _gather(b, b.rhs(ya), 1)) < 1e-13
+    a.step(2, use_graph=True)
+    b.step(2, use_graph=True)
+    assert _rel(_gather(a, a.get_state(), 1), _gather(b, b.get_state(), 1)) < 1e-13
+    # switching back re-captures the step with the fused launches
+    b.set_option("fuse_products", 1)
+    assert b.get_option("products_fused") == 1
+    a.step(1, use_graph=True)
+    b.step(1, use_graph=True)
+    assert _rel(_gather(a, a.get_state(), 1), _gather(b, b.get_state(), 1)) < 1e-13
+
+
+def test_products_option_where_the_pass_cannot_form_them(dev):
+    import dfft
+    s = _solver(24, 2, 2, 0)         # 24-point lines: no products kernel
+    assert s.get_option("fuse_products") == 1 and s.get_option("products_fused") == 0
+    t = _solver(16, 1, 1, 1)         # advective form: no products at all
+    assert t.get_option("products_fused") == 0
+    with pytest.raises(dfft.DFFTError):
+        s.set_option("fuse_products", 2)
+    with pytest.raises(dfft.DFFTError):
+        s.get_option("no_such_option")
+
+
 def test_graph_replay_matches_eager_bitwise(dev):
     a = _solver(32, 2, 2, 0)
     b = _solver(32, 2, 2, 0)
@@ -315,6 +353,7 @@ def test_taylor_green_256_cubed_2x2_production_run(dev):
     n, nu, dt = 256, 1.0 / 1600.0, 1e-3
     s = _solver(n, 2, 2, dfft_ns.DIVERGENCE, nu, dt)
     s.autotune()
+    assert s.get_option("products_fused") == 1
     s.init_taylor_green(1.0)
     # E(0) = pi^3 and the initial state
     assert abs(s.energy() - np.pi ** 3) < 1e-12 * np.pi ** 3
