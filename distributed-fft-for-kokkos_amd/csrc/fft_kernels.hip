@@ -275,7 +275,7 @@ namespace {
 // lines and the two spare slots (real / imaginary halves of each line)
 template <int N> void group_dual_shape_of(size_t cap, int* B, size_t* lds) {
   using Cfg = FastCfg<float, N, 8>;
-  const size_t line_bytes = line_lds_bytes<float, N, 8, true>();
+  const size_t line_bytes = dual_line_bytes<float, N, 8>();
   if (cap > kGroupLds) cap = kGroupLds;
   int pairs = kGroupThreads / Cfg::T;
   while (pairs > 1 && static_cast<size_t>(2 * pairs + 2) * line_bytes > cap) --pairs;

@@ -427,6 +427,19 @@ template <typename Real, int N, int RM, bool SPLIT> constexpr size_t line_lds_by
   return static_cast<size_t>(FastCfg<Real, N, RM>::LDSN) * (SPLIT ? sizeof(Real) : sizeof(cplx_t<Real>));
 }
 
+// Line buffer of the two-lines-per-lane kernel (reals): LDSN plus a pad
+// measured per length.  1024-point FP32 lines: one more float per line moves
+// the 16 line buffers a wave touches onto different banks (lane pairs read
+// and write 4 positions of each of 16 lines per access); the y pass of a
+// 1024^3 R2C plan took 1.80 instead of 2.08 ms with it, while the same pad
+// made 512-point lines 5-8 % slower.
+template <typename Real, int N> constexpr int dual_lds_extra() {
+  return (sizeof(Real) == 4 && N == 1024) ? 1 : 0;
+}
+template <typename Real, int N, int RM> constexpr size_t dual_line_bytes() {
+  return static_cast<size_t>(FastCfg<Real, N, RM>::LDSN + dual_lds_extra<Real, N>()) * sizeof(Real);
+}
+
 // Register <-> LDS permutation of one line: every lane stores its values
 // v[q*RW + j] at positions wpos(q, j), then loads v[q*RR + j] from positions
 // rpos(q, j) (NW*RW = NR*RR = R).  SPLIT moves the real parts first and the
@@ -627,7 +640,7 @@ __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, 
   using Cfg = FastCfg<Real, N, RM>;
   typedef Real V4 __attribute__((ext_vector_type(4)));
   constexpr int R = Cfg::R, T = Cfg::T;
-  constexpr size_t LB = line_lds_bytes<Real, N, RM, true>();
+  constexpr size_t LB = dual_line_bytes<Real, N, RM>();
   const int P2 = B >> 1;
   const int tid = threadIdx.x;
   // lanes beyond P2*T (grouped launches of 256 lanes) run the stages on zeros
@@ -883,7 +896,15 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
   void* buf = lds + b * line_lds_bytes<Real, N, RM, SPLIT>();
 
   PairIO<Real, NT> io{};
-  if (valid) io = make_pair_io<Real, NT, PROD>(p, pair);
+  if (T >= 64 && !p.lines_fast) {
+    // lanes along the pair with T >= 64: each wave holds one pair (or only
+    // spare lanes), so the pair index is wave-uniform and the descriptor
+    // arithmetic of make_pair_io runs once per wave on the scalar unit
+    const int32_t wp = __builtin_amdgcn_readfirstlane(pair);
+    if (valid) io = make_pair_io<Real, NT, PROD>(p, wp);
+  } else {
+    if (valid) io = make_pair_io<Real, NT, PROD>(p, pair);
+  }
 
   // Products pass: the block holds whole positions (B is a multiple of 3 and
   // the field axis is enumerated fastest, so slots 3m..3m+2 are the pairs
@@ -1254,7 +1275,7 @@ hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
     using Cfg = FastCfg<Real, N, RM>;
     constexpr int T = Cfg::T;
     const int P2 = B / 2;
-    const size_t smem = static_cast<size_t>(B) * line_lds_bytes<Real, N, RM, true>();
+    const size_t smem = static_cast<size_t>(B) * dual_line_bytes<Real, N, RM>();
     if (P2 * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
     p.lines_per_block = B;
     static std::atomic<size_t> lds_limit[4] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
