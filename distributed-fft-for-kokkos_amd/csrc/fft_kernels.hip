@@ -86,6 +86,175 @@ struct GenPlan {
   int32_t rad[32];
 };
 
+// The radix stages of B lines of length n held in bufA (line b at b*n):
+// every output of a radix-r stage is an r-term sum with combined twiddles
+// (tw: exp(-2 pi i e / n)).  Returns the buffer that holds the result.
+template <typename Real>
+__device__ cplx_t<Real>* generic_stages(cplx_t<Real>* bufA, cplx_t<Real>* bufB, int32_t n, int32_t total,
+                                        const GenPlan& g, const double2* tw) {
+  using C = cplx_t<Real>;
+  int32_t L = 1;
+  for (int s = 0; s < g.nst; ++s) {
+    const int32_t r = g.rad[s];
+    const int32_t Lr = L * r;
+    const int32_t ts = n / Lr;
+    const int32_t nr = n / r;
+    for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
+      const int32_t b = e / n, o = e - b * n;
+      const int32_t grp = o / Lr, rm = o - grp * Lr;
+      const int32_t q = rm / L, k = rm - q * L;
+      const int32_t beta = grp * L + k;
+      const C* src = bufA + b * n;
+      C acc{Real(0), Real(0)};
+      const int32_t step = k + q * L;  // exponent step in units of W_{Lr}
+      int32_t ex = 0;
+      for (int32_t j = 0; j < r; ++j) {
+        C x = src[beta + j * nr];
+        acc = cadd(acc, cmul(x, to_prec<C>(tw[ex * ts])));
+        ex += step;
+        if (ex >= Lr) ex -= Lr * (ex / Lr);
+      }
+      bufB[e] = acc;
+    }
+    __syncthreads();
+    C* tmp = bufA;
+    bufA = bufB;
+    bufB = tmp;
+    L = Lr;
+  }
+  return bufA;
+}
+
+// Input value at position pos of line `line` of pass p (per-element
+// addressing): R2C lines load reals, C2R lines the Hermitian extension of
+// the half spectrum (with the interior weight), inverse passes conjugate.
+template <typename Real>
+__device__ __forceinline__ cplx_t<Real> generic_load(const LinePass& p, int32_t line, int32_t pos) {
+  using C = cplx_t<Real>;
+  const int32_t n = p.n;
+  int32_t idx[kMaxDim];
+  decode_line<Real>(p, line, idx);
+  const bool mirror = p.kind == kC2R && pos >= n / 2 + 1;
+  idx[p.axis] = mirror ? n - pos : pos;
+  int64_t st;
+  const int64_t ies = p.kind == kR2C ? sizeof(Real) : sizeof(C);
+  const char* ib = element_ptr(p.in, idx, ies, p.axis, st);
+  C v;
+  if (p.kind == kR2C) {
+    v = C{*reinterpret_cast<const Real*>(ib), Real(0)};
+  } else {
+    v = *reinterpret_cast<const C*>(ib);
+  }
+  if (p.kind == kC2R) {
+    if (mirror) v = cconj(v);
+    if (pos == 0 || 2 * pos == n) {
+      v.y = Real(0);
+    } else if (p.herm_weight != 0.0) {
+      v.x *= static_cast<Real>(p.herm_weight);
+      v.y *= static_cast<Real>(p.herm_weight);
+    }
+  }
+  return p.inverse ? cconj(v) : v;
+}
+
+// Store of the transform output v (before the inverse conjugation) at
+// position pos of line `line`: scale, R2C keeps pos <= n/2 (interior weight),
+// C2R stores the real part.
+template <typename Real>
+__device__ __forceinline__ void generic_store(const LinePass& p, int32_t line, int32_t pos, cplx_t<Real> v) {
+  using C = cplx_t<Real>;
+  const int32_t n = p.n;
+  if (p.kind == kR2C && 2 * pos > n) return;
+  if (p.inverse) v = cconj(v);
+  Real sc = static_cast<Real>(p.scale);
+  if (p.kind == kR2C && pos != 0 && 2 * pos != n && p.herm_weight != 0.0) sc *= static_cast<Real>(p.herm_weight);
+  int32_t idx[kMaxDim];
+  decode_line<Real>(p, line, idx);
+  idx[p.axis] = pos;
+  int64_t st;
+  const int64_t oes = p.kind == kC2R ? sizeof(Real) : sizeof(C);
+  char* ob = element_ptr(p.out, idx, oes, p.axis, st);
+  if (p.kind == kC2R) {
+    *reinterpret_cast<Real*>(ob) = v.x * sc;
+  } else {
+    *reinterpret_cast<C*>(ob) = C{v.x * sc, v.y * sc};
+  }
+}
+
+// Lines longer than one LDS buffer (LinePass::long_n1 > 0, n = n1 * n2),
+// first launch: sub-line (line, i2) holds elements i1*n2 + i2 (i1 < n1) of
+// the line; its transform Y(k1) times exp(-2 pi i i2 k1 / n) goes to
+// scratch[(line*n1 + k1)*n2 + i2].  Lanes walk the B sub-lines of a block
+// fastest (consecutive i2: consecutive elements of the line on both sides).
+template <typename Real>
+__global__ void __launch_bounds__(256) fft_long_first(const LinePass p, const GenPlan g) {
+  using C = cplx_t<Real>;
+  extern __shared__ unsigned char smem_raw[];
+  const int32_t n = p.n, n1 = p.long_n1, n2 = n / n1;
+  const int B = p.lines_per_block;
+  C* bufA = reinterpret_cast<C*>(smem_raw);
+  C* bufB = bufA + B * n1;
+  const int32_t total = B * n1;
+  const int64_t nsub = static_cast<int64_t>(p.nlines) * n2;
+  const int64_t sub0 = static_cast<int64_t>(blockIdx.x) * B;
+  for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int32_t b = e % B, i1 = e / B;
+    const int64_t sub = sub0 + b;
+    C v{Real(0), Real(0)};
+    if (sub < nsub) {
+      const int32_t line = static_cast<int32_t>(sub / n2), i2 = static_cast<int32_t>(sub % n2);
+      if (!padding_line<Real>(p, line)) v = generic_load<Real>(p, line, i1 * n2 + i2);
+    }
+    bufA[b * n1 + i1] = v;
+  }
+  __syncthreads();
+  C* res = generic_stages<Real>(bufA, bufB, n1, total, g, reinterpret_cast<const double2*>(p.long_tw1));
+  const double2* twn = reinterpret_cast<const double2*>(p.tw);
+  C* scr = reinterpret_cast<C*>(p.long_scratch);
+  for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int32_t b = e % B, k1 = e / B;
+    const int64_t sub = sub0 + b;
+    if (sub >= nsub) continue;
+    const int32_t line = static_cast<int32_t>(sub / n2), i2 = static_cast<int32_t>(sub % n2);
+    // i2 * k1 <= (n2 - 1)(n1 - 1) < n: a table index without reduction
+    const C w = to_prec<C>(twn[i2 * k1]);
+    scr[(static_cast<int64_t>(line) * n1 + k1) * n2 + i2] = cmul(res[b * n1 + k1], w);
+  }
+}
+
+// Second launch: sub-line (line, k1) is scratch[(line*n1 + k1)*n2 + i2]
+// (i2 < n2, contiguous); output k2 of its transform is element k1 + n1*k2 of
+// the line.  Loads walk each sub-line, stores walk the sub-lines fastest
+// (consecutive k1: consecutive elements of the output line).
+template <typename Real>
+__global__ void __launch_bounds__(256) fft_long_second(const LinePass p, const GenPlan g) {
+  using C = cplx_t<Real>;
+  extern __shared__ unsigned char smem_raw[];
+  const int32_t n = p.n, n1 = p.long_n1, n2 = n / n1;
+  const int B = p.lines_per_block;
+  C* bufA = reinterpret_cast<C*>(smem_raw);
+  C* bufB = bufA + B * n2;
+  const int32_t total = B * n2;
+  const int64_t nsub = static_cast<int64_t>(p.nlines) * n1;
+  const int64_t sub0 = static_cast<int64_t>(blockIdx.x) * B;
+  const C* scr = reinterpret_cast<const C*>(p.long_scratch);
+  for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int32_t b = e / n2, i2 = e % n2;
+    const int64_t sub = sub0 + b;
+    bufA[e] = sub < nsub ? scr[sub * n2 + i2] : C{Real(0), Real(0)};
+  }
+  __syncthreads();
+  C* res = generic_stages<Real>(bufA, bufB, n2, total, g, reinterpret_cast<const double2*>(p.long_tw2));
+  for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const int32_t b = e % B, k2 = e / B;
+    const int64_t sub = sub0 + b;
+    if (sub >= nsub) continue;
+    const int32_t line = static_cast<int32_t>(sub / n1), k1 = static_cast<int32_t>(sub % n1);
+    if (padding_line<Real>(p, line)) continue;
+    generic_store<Real>(p, line, k1 + n1 * k2, res[b * n2 + k2]);
+  }
+}
+
 template <typename Real>
 __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const GenPlan g) {
   using C = cplx_t<Real>;
@@ -139,35 +308,7 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
     bufA[e] = v;
   }
   __syncthreads();
-  int32_t L = 1;
-  for (int s = 0; s < g.nst; ++s) {
-    const int32_t r = g.rad[s];
-    const int32_t Lr = L * r;
-    const int32_t ts = n / Lr;
-    const int32_t nr = n / r;
-    for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
-      const int32_t b = e / n, o = e - b * n;
-      const int32_t grp = o / Lr, rm = o - grp * Lr;
-      const int32_t q = rm / L, k = rm - q * L;
-      const int32_t beta = grp * L + k;
-      const C* src = bufA + b * n;
-      C acc{Real(0), Real(0)};
-      const int32_t step = k + q * L;  // exponent step in units of W_{Lr}
-      int32_t ex = 0;
-      for (int32_t j = 0; j < r; ++j) {
-        C x = src[beta + j * nr];
-        acc = cadd(acc, cmul(x, to_prec<C>(tw[ex * ts])));
-        ex += step;
-        if (ex >= Lr) ex -= Lr * (ex / Lr);
-      }
-      bufB[e] = acc;
-    }
-    __syncthreads();
-    C* tmp = bufA;
-    bufA = bufB;
-    bufB = tmp;
-    L = Lr;
-  }
+  bufA = generic_stages<Real>(bufA, bufB, n, total, g, tw);
   for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
     const int32_t b = e / n, pos = e - b * n;
     const int32_t line = blockIdx.x * B + b;
@@ -200,12 +341,10 @@ __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const 
 // ---------------------------------------------------------------------------
 namespace {
 
-template <typename Real>
-hipError_t launch_generic(LinePass p, hipStream_t s) {
-  using C = cplx_t<Real>;
-  GenPlan g{};
-  int m = p.n;
-  g.nst = 0;
+// radices of the generic kernel for length n: 4s, then prime factors
+bool make_gen_plan(int n, GenPlan& g) {
+  g = GenPlan{};
+  int m = n;
   while (m % 4 == 0 && m > 4) { g.rad[g.nst++] = 4; m /= 4; }
   for (int f = 2; m > 1 && g.nst < 32;) {
     if (m % f == 0) {
@@ -216,7 +355,58 @@ hipError_t launch_generic(LinePass p, hipStream_t s) {
       if (f * f > m) f = m;
     }
   }
-  if (m != 1) return hipErrorInvalidValue;
+  return m == 1;
+}
+
+// sub-lines per block of a long-line launch: lines of length len, twice, in
+// at most 64 KB of LDS (at least one line, up to 160 KB)
+template <typename Real>
+int long_block(int len) {
+  const size_t line = static_cast<size_t>(2) * len * sizeof(cplx_t<Real>);
+  int B = 64;
+  while (B > 1 && static_cast<size_t>(B) * line > 64 * 1024) B /= 2;
+  return B;
+}
+
+template <typename Real>
+hipError_t launch_long(LinePass p, hipStream_t s) {
+  using C = cplx_t<Real>;
+  const int n1 = p.long_n1;
+  if (n1 <= 0 || p.n % n1 != 0 || !p.long_scratch || !p.long_tw1 || !p.long_tw2) return hipErrorInvalidValue;
+  const int n2 = p.n / n1;
+  GenPlan g1, g2;
+  if (!make_gen_plan(n1, g1) || !make_gen_plan(n2, g2)) return hipErrorInvalidValue;
+  static std::atomic<size_t> lds1{64 * 1024}, lds2{64 * 1024};
+  {
+    LinePass q = p;
+    q.lines_per_block = long_block<Real>(n1);
+    const size_t smem = static_cast<size_t>(2) * q.lines_per_block * n1 * sizeof(C);
+    if (smem > kGenericLdsMax) return hipErrorInvalidValue;
+    const hipError_t e = raise_lds_limit(lds1, reinterpret_cast<const void*>(&fft_long_first<Real>), smem);
+    if (e != hipSuccess) return e;
+    const int64_t grid = (static_cast<int64_t>(p.nlines) * n2 + q.lines_per_block - 1) / q.lines_per_block;
+    if (grid >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((fft_long_first<Real>), dim3(static_cast<unsigned>(grid)), dim3(256), smem, s, q, g1);
+    const hipError_t e1 = hipGetLastError();
+    if (e1 != hipSuccess) return e1;
+  }
+  LinePass q = p;
+  q.lines_per_block = long_block<Real>(n2);
+  const size_t smem = static_cast<size_t>(2) * q.lines_per_block * n2 * sizeof(C);
+  if (smem > kGenericLdsMax) return hipErrorInvalidValue;
+  const hipError_t e = raise_lds_limit(lds2, reinterpret_cast<const void*>(&fft_long_second<Real>), smem);
+  if (e != hipSuccess) return e;
+  const int64_t grid = (static_cast<int64_t>(p.nlines) * n1 + q.lines_per_block - 1) / q.lines_per_block;
+  if (grid >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((fft_long_second<Real>), dim3(static_cast<unsigned>(grid)), dim3(256), smem, s, q, g2);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_generic(LinePass p, hipStream_t s) {
+  using C = cplx_t<Real>;
+  GenPlan g{};
+  if (!make_gen_plan(p.n, g)) return hipErrorInvalidValue;
   int B = p.n >= 1024 ? 1 : 1024 / p.n;
   while (B > 1 && static_cast<size_t>(2) * B * p.n * sizeof(C) > 64 * 1024) B /= 2;
   const size_t smem = static_cast<size_t>(2) * B * p.n * sizeof(C);
@@ -354,11 +544,23 @@ bool has_fast_line_kernel(int n) {
   }
 }
 
-bool line_length_supported(int n, bool fp64) {
+bool line_length_single_pass(int n, bool fp64) {
   if (n < 1) return false;
   if (has_fast_line_kernel(n)) return true;
   const size_t csize = fp64 ? sizeof(double2) : sizeof(float2);
   return static_cast<size_t>(2) * static_cast<size_t>(n) * csize <= kGenericLdsMax;
+}
+
+int long_line_split(int n, bool fp64) {
+  if (n < 4 || line_length_single_pass(n, fp64)) return 0;
+  int best = 0;
+  for (int d = 2; static_cast<int64_t>(d) * d <= n; ++d)
+    if (n % d == 0 && line_length_single_pass(d, fp64) && line_length_single_pass(n / d, fp64)) best = d;
+  return best;
+}
+
+bool line_length_supported(int n, bool fp64) {
+  return line_length_single_pass(n, fp64) || long_line_split(n, fp64) > 0;
 }
 
 bool products_pass_supported(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
@@ -372,6 +574,7 @@ hipError_t launch_products_pass(const LinePass& p, hipStream_t s) {
 
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s) {
   if (p.nlines <= 0) return hipSuccess;
+  if (p.long_n1 > 0) return fp64 ? launch_long<double>(p, s) : launch_long<float>(p, s);
   // the compile-time kernel needs the line axis undistributed on both sides
   const bool fast = has_fast_line_kernel(p.n) && p.in.ntopo[p.axis] == 1 && p.out.ntopo[p.axis] == 1;
   if (fast) return fp64 ? dispatch<double>(p, s) : dispatch<float>(p, s);

@@ -75,6 +75,15 @@ struct LinePass {
   // C2R line, applied to the values a C2R line loads or an R2C line stores;
   // 0 means 1.  Adjoint transforms use 1/2 (C2R) and 2 (R2C).
   double herm_weight;
+  // long_n1 > 0: lines too long for one LDS buffer, n = n1 * n2, run as two
+  // launches through long_scratch (complex, nlines * n values): first the n2
+  // sub-lines of stride n2 and length n1 of every line, each output k1
+  // multiplied by exp(-2 pi i i2 k1 / n); then the n1 contiguous sub-lines of
+  // length n2, whose output k2 is element k1 + n1 k2 of the line
+  int32_t long_n1;
+  const void* long_tw1;  // exp(-2 pi i e / n1)
+  const void* long_tw2;  // exp(-2 pi i e / n2)
+  void* long_scratch;
 };
 
 // Element-wise permuted copy between two distributions of the same logical
@@ -108,8 +117,13 @@ int group_grid(size_t lds_bytes, bool fp64, bool dual);
 hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
                              int grid, size_t lds_bytes, bool fp64, bool nontemporal, bool dual, hipStream_t s);
 bool has_fast_line_kernel(int n);
-// true if a line of length n has a kernel: the fast set, or the generic
+// true if a line of length n runs as one pass: the fast set, or the generic
 // kernel (one line of n complex values, twice, in at most 160 KB of LDS)
+bool line_length_single_pass(int n, bool fp64);
+// n1 of a two-pass split n = n1 * n2 with both factors single-pass lengths
+// (the divisor nearest sqrt(n), n1 <= n2), 0 if n has none
+int long_line_split(int n, bool fp64);
+// single pass or a two-pass split
 bool line_length_supported(int n, bool fp64);
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
 // an FP64 R2C pass with p.products set (paired lines, lengths 8..1024 powers

@@ -104,7 +104,7 @@ Group::Group(int count, const dfft_desc* descs) {
       }
       for (int a : axes) {
         const int len = static_cast<int>(m.gin[a]);
-        if (!line_length_supported(len, fp64_))
+        if (!line_length_single_pass(len, fp64_))
           throw Error(DFFT_ERROR_UNSUPPORTED, who + "no line kernel for transform length " + std::to_string(len));
       }
       for (int a : axes) {

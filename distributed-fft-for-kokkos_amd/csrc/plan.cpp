@@ -136,7 +136,7 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
     if (!line_length_supported(static_cast<int>(n), fp64_))
       throw Error(DFFT_ERROR_UNSUPPORTED, "no line kernel for transform length " + std::to_string(n) +
                                               (fp64_ ? " in FP64" : " in FP32") +
-                                              " (lengths outside the fast set must fit one line in LDS; DFFT_EXEC_VENDOR takes any length)");
+                                              " (a length needs a fast kernel, one line in LDS, or two such factors; DFFT_EXEC_VENDOR takes any length)");
   }
 
   check_hip(hipGetDevice(&device_), "hipGetDevice");
@@ -154,6 +154,9 @@ void Plan::release() {
   tw_.clear();
   if (scratch_) (void)hipFree(scratch_);
   scratch_ = nullptr;
+  if (long_scratch_) (void)hipFree(long_scratch_);
+  long_scratch_ = nullptr;
+  long_bytes_ = 0;
   for (auto& v : walk_bufs_) {
     for (void* p : v) (void)hipFree(p);
     v.clear();
@@ -171,14 +174,28 @@ void Plan::allocate() {
   // twiddle tables exp(-2 pi i e / n), evaluated in long double and rounded
   // once to FP64 -- for both precisions: the kernels form the powers of a
   // butterfly in FP64 and round each once to the compute precision
+  bool long_lines = false;
   for (std::size_t k = 0; k < axes_.size(); ++k) {
     const int n = static_cast<int>(fft_extents_[k]);
-    if (tw_.count(n)) continue;
-    tw_[n] = upload_twiddles(n);
-    workspace_bytes_ += 16 * static_cast<std::size_t>(n);
+    const int n1 = long_line_split(n, fp64_);
+    for (int len : {n, n1, n1 > 0 ? n / n1 : 0}) {
+      if (len <= 0 || tw_.count(len)) continue;
+      tw_[len] = upload_twiddles(len);
+      workspace_bytes_ += 16 * static_cast<std::size_t>(len);
+    }
+    if (n1 > 0) long_lines = true;
   }
 
   const std::size_t cs = fp64_ ? 16 : 8;
+  if (long_lines && total_size(gout_) > 0) {
+    // every line of a pass, complex and whole, over the padded scratch rows
+    const std::size_t elems =
+        std::max({scratch_elems(gout_, cs), scratch_elems(gin_, cs), scratch_elems(gin_, cs / 2)});
+    long_bytes_ = elems * cs;
+    hipError_t err = hipMalloc(&long_scratch_, long_bytes_);
+    if (err != hipSuccess) throw Error(DFFT_ERROR_OUT_OF_MEMORY, "hipMalloc of the long-line scratch failed");
+    workspace_bytes_ += long_bytes_;
+  }
   if (d_.exec_mode == DFFT_EXEC_FUSED) {
     plan_fused();
     // the spectrum (gout, complex) and, for an in-place backward call, the
@@ -425,6 +442,16 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   }
   if (nl >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many lines in one pass");
   p.nlines = static_cast<int32_t>(nl);
+  const int n1 = long_line_split(s.n, fp64_);
+  if (n1 > 0) {
+    const std::size_t need = static_cast<std::size_t>(nl) * static_cast<std::size_t>(s.n) * (fp64_ ? 16 : 8);
+    if (!long_scratch_ || need > long_bytes_)
+      throw Error(DFFT_ERROR_UNSUPPORTED, "long-line scratch too small for a pass of " + std::to_string(nl) + " lines");
+    p.long_n1 = n1;
+    p.long_tw1 = twiddles(n1);
+    p.long_tw2 = twiddles(s.n / n1);
+    p.long_scratch = long_scratch_;
+  }
   p.line_begin = 0;
   p.lines_fast = s.lines_fast >= 0 ? s.lines_fast : (s.axis != innermost ? 1 : 0);
   if (s.nontemporal >= 0) {

@@ -118,6 +118,10 @@ class Plan {
   std::vector<BlockAnalyses> analyses_;
   std::size_t max_buffer_size_ = 0;
   void* scratch_ = nullptr;
+  // lines longer than one LDS buffer: complex scratch of the two-launch
+  // transform (nlines * n values of a pass)
+  void* long_scratch_ = nullptr;
+  std::size_t long_bytes_ = 0;
   std::vector<void*> walk_bufs_[2];  // block-walk ping-pong local arrays per rank
   std::unique_ptr<VendorFFT> vendor_;  // DFFT_EXEC_VENDOR: rocFFT over the gathered global array
   std::size_t workspace_bytes_ = 0;

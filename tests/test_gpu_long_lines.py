@@ -1,0 +1,94 @@
+"""Transform lengths longer than one line in LDS (n = n1 * n2, two launches
+through a complex scratch) against the oracle, in every executor that runs
+them natively, and their adjoints.  The reference's DynPlan takes any
+extent (SlabPlan.hpp:1098-1113)."""
+
+import numpy as np
+import pytest
+
+from helpers import gather_rel_l2, out_extents, reference_spectrum, run_gpu_plan, seeded_input
+from oracle import plan_exec as PE
+from oracle import planner as PL
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # gin, axes, tin, tout, fp64
+    ([8192, 6, 4], [0, 1, 2], [1, 2, 1], [1, 1, 2], True),     # 8192 = 64 x 128, C2C axis first
+    ([4, 6, 8192], [0, 1, 2], [2, 1, 1], [1, 2, 1], True),     # the long axis is the R2C axis
+    ([6405, 3, 4], [0, 1, 2], [1, 1, 2], [1, 2, 1], True),     # 6405 = 61 x 105 (odd radices, prime 61)
+    ([6, 8, 8192], [2, 1, 0], [1, 2, 2], [2, 2, 1], True),     # pencil, long axis transformed first
+    ([2, 3, 12288], [0, 1, 2], [2, 1, 1], [1, 2, 1], False),   # FP32: 12288 = 96 x 128
+    ([12288, 4], [0, 1], [1, 2], [2, 1], False),               # 2-D view
+]
+
+
+@pytest.mark.parametrize("gin,axes,tin,tout,fp64", CASES)
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("exec_mode", [0, 1])
+def test_long_lines_match_oracle(dev, gin, axes, tin, tout, fp64, real, exec_mode):
+    x = seeded_input(gin, real, 3)
+    outs, backs, ins, plan = run_gpu_plan(dev, x, gin, axes, tin, tout, real, fp64, "backward", exec_mode)
+    ref = reference_spectrum(x, gin, axes, tout, real)
+    tol = 1e-12 if fp64 else 1e-5
+    e_fwd, e_rt = gather_rel_l2(outs, ref), gather_rel_l2(backs, ins)
+    assert e_fwd <= tol, f"forward rel-L2 {e_fwd:.3e}"
+    assert e_rt <= tol, f"round trip rel-L2 {e_rt:.3e}"
+
+
+def test_long_line_closed_form(dev):
+    """A single mode exp(2 pi i m j / n) along an 8192-point axis transforms
+    to n at k = m and 0 elsewhere (independent of the oracle's FFT)."""
+    n, m = 8192, 1237
+    j = np.arange(n)
+    x = np.broadcast_to(np.exp(2j * np.pi * m * j / n)[:, None], (n, 4)).copy()
+    outs, _, _, _ = run_gpu_plan(dev, x, [n, 4], [0, 1], [1, 2], [2, 1], False, True)
+    got = PE.gather(outs, out_extents([n, 4], [0, 1], False), [2, 1], np.complex128)
+    want = np.zeros((n, 4), np.complex128)
+    want[m, 0] = n * 4
+    assert np.abs(got - want).max() <= 1e-9 * n * 4
+
+
+def test_prime_beyond_one_line_is_refused(dev):
+    import torch
+    import dfft
+    n = 6007  # prime, 2 * 6007 complex128 values exceed the LDS of one workgroup
+    ins = [torch.zeros(n, 2, dtype=torch.complex128, device=dev) for _ in range(2)]
+    outs = [torch.zeros(n // 2 + (1 - r), 4, dtype=torch.complex128, device=dev) for r in range(2)]
+    with pytest.raises(dfft.DFFTError) as e:
+        dfft.Plan(ins, outs, [0, 1], [1, 2], [2, 1], global_in_extents=[n, 4], global_out_extents=[n, 4])
+    assert e.value.code == dfft.ERROR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("real", [False, True])
+def test_long_line_adjoint_dot_product(dev, real):
+    """<F x, y> = <x, F^* y> with a 8192-point axis (R2C axis when real)."""
+    import torch
+    import dfft
+    gin, axes, tin, tout = [4, 6, 8192], [0, 1, 2], [2, 1, 1], [1, 2, 1]
+    gout = out_extents(gin, axes, real)
+    cdt = torch.complex128
+    idt = torch.float64 if real else cdt
+
+    def blocks(arr, ext, topo, dt):
+        return [torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt) for a in PE.scatter(arr, ext, list(topo))]
+
+    def rdot(a, b):
+        return sum(float((np.conj(p.cpu().numpy()) * q.cpu().numpy()).real.sum()) for p, q in zip(a, b))
+
+    def norm(a):
+        return float(np.sqrt(sum(float(np.linalg.norm(t.cpu().numpy().ravel())) ** 2 for t in a)))
+
+    ins = [torch.zeros(PL.compute_local_extents_and_starts(gin, tin, r, PL.LAYOUT_RIGHT)[0], dtype=idt, device=dev)
+           for r in range(2)]
+    outs = [torch.zeros(PL.compute_local_extents_and_starts(gout, tout, r, PL.LAYOUT_RIGHT)[0], dtype=cdt,
+                        device=dev) for r in range(2)]
+    plan = dfft.Plan(ins, outs, axes, tin, tout, global_in_extents=gin, global_out_extents=gout)
+    x = blocks(seeded_input(gin, real, 1), gin, tin, idt)
+    y = blocks(seeded_input(gout, False, 2), gout, tout, cdt)
+    fx = [torch.empty_like(t) for t in y]
+    fty = [torch.empty_like(t) for t in x]
+    plan._execute(x, fx, dfft.Direction.forward)
+    plan.execute_adjoint(y, fty, dfft.Direction.forward)
+    lhs, rhs = rdot(fx, y), rdot(x, fty)
+    assert abs(lhs - rhs) <= 1e-12 * (norm(x) * norm(fty) + norm(fx) * norm(y)), (lhs, rhs)
