@@ -16,6 +16,7 @@
 #include "fft_kernels.hpp"
 #include "fft_line_impl.hpp"
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -255,6 +256,62 @@ __global__ void __launch_bounds__(256) fft_long_second(const LinePass p, const G
   }
 }
 
+// Bluestein (chirp-z) transform of lines whose length has a prime factor
+// beyond one LDS line: with c_e = exp(-pi i e / n) (e = j^2 mod 2n),
+//   X_k = c_{k^2} sum_j (x_j c_{j^2}) conj(c_{(k-j)^2}),
+// a cyclic convolution of length m >= 2n - 1 (a power of two) carried out
+// with two m-point transforms of the (nlines, m) scratch between these
+// kernels.  bluestein_pre: s1[line][j] = x_j c_{j^2} (j < n), 0 (n <= j < m);
+// the loads follow the generic kernel (R2C reals, C2R Hermitian extension,
+// inverse conjugation).
+template <typename Real>
+__global__ void __launch_bounds__(256) bluestein_pre(const LinePass p, int32_t log2m, const double2* chirp,
+                                                     cplx_t<Real>* s1) {
+  using C = cplx_t<Real>;
+  const int64_t m = int64_t(1) << log2m;
+  const int64_t total = static_cast<int64_t>(p.nlines) << log2m;
+  const int64_t n2 = 2 * static_cast<int64_t>(p.n);
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t line = static_cast<int32_t>(g >> log2m);
+    const int64_t j = g & (m - 1);
+    C v{Real(0), Real(0)};
+    if (j < p.n && !padding_line<Real>(p, line)) {
+      const C x = generic_load<Real>(p, line, static_cast<int32_t>(j));
+      v = cmul(x, to_prec<C>(chirp[(j * j) % n2]));
+    }
+    s1[g] = v;
+  }
+}
+
+// s[line][k] *= filt[k]: the m-point transform of the chirp filter, divided by m
+template <typename Real>
+__global__ void __launch_bounds__(256) bluestein_mul(int64_t total, int32_t log2m, const double2* filt,
+                                                     cplx_t<Real>* s) {
+  using C = cplx_t<Real>;
+  const int64_t mask = (int64_t(1) << log2m) - 1;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    s[g] = cmul(s[g], to_prec<C>(filt[g & mask]));
+}
+
+// X_k = c_{k^2} s1[line][k] (k < n), stored as the generic kernel stores
+template <typename Real>
+__global__ void __launch_bounds__(256) bluestein_post(const LinePass p, int32_t log2m, const double2* chirp,
+                                                      const cplx_t<Real>* s1) {
+  using C = cplx_t<Real>;
+  const int64_t total = static_cast<int64_t>(p.nlines) * p.n;
+  const int64_t n2 = 2 * static_cast<int64_t>(p.n);
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t line = static_cast<int32_t>(g / p.n);
+    const int64_t k = g - static_cast<int64_t>(line) * p.n;
+    if (padding_line<Real>(p, line)) continue;
+    const C y = s1[(static_cast<int64_t>(line) << log2m) + k];
+    generic_store<Real>(p, line, static_cast<int32_t>(k), cmul(y, to_prec<C>(chirp[(k * k) % n2])));
+  }
+}
+
 template <typename Real>
 __global__ void __launch_bounds__(256) fft_line_generic(const LinePass p, const GenPlan g) {
   using C = cplx_t<Real>;
@@ -400,6 +457,11 @@ hipError_t launch_long(LinePass p, hipStream_t s) {
   if (grid >= (int64_t(1) << 31)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((fft_long_second<Real>), dim3(static_cast<unsigned>(grid)), dim3(256), smem, s, q, g2);
   return hipGetLastError();
+}
+
+unsigned elementwise_grid(int64_t total) {
+  const int64_t blocks = (total + 255) / 256;
+  return static_cast<unsigned>(std::min<int64_t>(blocks, 65536));
 }
 
 template <typename Real>
@@ -570,6 +632,37 @@ hipError_t launch_products_pass(const LinePass& p, hipStream_t s) {
   if (!products_pass_supported(p.n) || p.in.ntopo[p.axis] != 1 || p.out.ntopo[p.axis] != 1)
     return hipErrorNotSupported;
   return dispatch_products_f64(p, s);
+}
+
+hipError_t launch_bluestein_pre(const LinePass& p, int log2m, const void* chirp, void* s1, bool fp64,
+                                hipStream_t s) {
+  if (p.nlines <= 0) return hipSuccess;
+  const unsigned grid = elementwise_grid(static_cast<int64_t>(p.nlines) << log2m);
+  const double2* c = static_cast<const double2*>(chirp);
+  if (fp64) hipLaunchKernelGGL((bluestein_pre<double>), dim3(grid), dim3(256), 0, s, p, log2m, c, static_cast<double2*>(s1));
+  else hipLaunchKernelGGL((bluestein_pre<float>), dim3(grid), dim3(256), 0, s, p, log2m, c, static_cast<float2*>(s1));
+  return hipGetLastError();
+}
+
+hipError_t launch_bluestein_mul(int64_t total, int log2m, const void* filt, void* buf, bool fp64, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  const unsigned grid = elementwise_grid(total);
+  const double2* f = static_cast<const double2*>(filt);
+  if (fp64) hipLaunchKernelGGL((bluestein_mul<double>), dim3(grid), dim3(256), 0, s, total, log2m, f, static_cast<double2*>(buf));
+  else hipLaunchKernelGGL((bluestein_mul<float>), dim3(grid), dim3(256), 0, s, total, log2m, f, static_cast<float2*>(buf));
+  return hipGetLastError();
+}
+
+hipError_t launch_bluestein_post(const LinePass& p, int log2m, const void* chirp, const void* s1, bool fp64,
+                                 hipStream_t s) {
+  if (p.nlines <= 0) return hipSuccess;
+  const unsigned grid = elementwise_grid(static_cast<int64_t>(p.nlines) * p.n);
+  const double2* c = static_cast<const double2*>(chirp);
+  if (fp64)
+    hipLaunchKernelGGL((bluestein_post<double>), dim3(grid), dim3(256), 0, s, p, log2m, c, static_cast<const double2*>(s1));
+  else
+    hipLaunchKernelGGL((bluestein_post<float>), dim3(grid), dim3(256), 0, s, p, log2m, c, static_cast<const float2*>(s1));
+  return hipGetLastError();
 }
 
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s) {

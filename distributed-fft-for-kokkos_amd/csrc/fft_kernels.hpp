@@ -126,6 +126,13 @@ int long_line_split(int n, bool fp64);
 // single pass or a two-pass split
 bool line_length_supported(int n, bool fp64);
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
+// Bluestein steps of pass p over an (nlines, 2^log2m) complex scratch (see
+// fft_kernels.hip); chirp: exp(-pi i e / n), e in [0, 2n), FP64; filt: the
+// m-point transform of the chirp filter divided by m, FP64
+hipError_t launch_bluestein_pre(const LinePass& p, int log2m, const void* chirp, void* s1, bool fp64, hipStream_t s);
+hipError_t launch_bluestein_mul(int64_t total, int log2m, const void* filt, void* buf, bool fp64, hipStream_t s);
+hipError_t launch_bluestein_post(const LinePass& p, int log2m, const void* chirp, const void* s1, bool fp64,
+                                 hipStream_t s);
 // an FP64 R2C pass with p.products set (paired lines, lengths 8..1024 powers
 // of two); hipErrorNotSupported for other passes
 hipError_t launch_products_pass(const LinePass& p, hipStream_t s);

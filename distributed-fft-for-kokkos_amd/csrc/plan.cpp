@@ -131,12 +131,12 @@ Plan::Plan(const dfft_desc& d) : d_(d) {
   // the TplPlan role (TplPlan.hpp:25-181): the vendor library names the plan
   if (d.exec_mode == DFFT_EXEC_VENDOR) name_ = "rocFFTPlan";
 
+  // every length runs: the fast set, one line in LDS, two such factors, or
+  // Bluestein over a power-of-two convolution m >= 2n - 1
   for (std::size_t k = 0; k < axes_.size() && d.exec_mode != DFFT_EXEC_VENDOR; ++k) {
     const std::size_t n = fft_extents_[k];
-    if (!line_length_supported(static_cast<int>(n), fp64_))
-      throw Error(DFFT_ERROR_UNSUPPORTED, "no line kernel for transform length " + std::to_string(n) +
-                                              (fp64_ ? " in FP64" : " in FP32") +
-                                              " (a length needs a fast kernel, one line in LDS, or two such factors; DFFT_EXEC_VENDOR takes any length)");
+    if (!line_length_supported(static_cast<int>(n), fp64_) && n > (std::size_t(1) << 27))
+      throw Error(DFFT_ERROR_UNSUPPORTED, "transform length " + std::to_string(n) + " too long for Bluestein");
   }
 
   check_hip(hipGetDevice(&device_), "hipGetDevice");
@@ -157,6 +157,16 @@ void Plan::release() {
   if (long_scratch_) (void)hipFree(long_scratch_);
   long_scratch_ = nullptr;
   long_bytes_ = 0;
+  for (auto& kv : blue_) {
+    if (kv.second.chirp) (void)hipFree(kv.second.chirp);
+    if (kv.second.filt) (void)hipFree(kv.second.filt);
+  }
+  blue_.clear();
+  for (void*& b : blue_s_) {
+    if (b) (void)hipFree(b);
+    b = nullptr;
+  }
+  blue_bytes_ = 0;
   for (auto& v : walk_bufs_) {
     for (void* p : v) (void)hipFree(p);
     v.clear();
@@ -174,27 +184,52 @@ void Plan::allocate() {
   // twiddle tables exp(-2 pi i e / n), evaluated in long double and rounded
   // once to FP64 -- for both precisions: the kernels form the powers of a
   // butterfly in FP64 and round each once to the compute precision
-  bool long_lines = false;
-  for (std::size_t k = 0; k < axes_.size(); ++k) {
-    const int n = static_cast<int>(fft_extents_[k]);
-    const int n1 = long_line_split(n, fp64_);
-    for (int len : {n, n1, n1 > 0 ? n / n1 : 0}) {
+  const std::size_t cs = fp64_ ? 16 : 8;
+  // elements of the largest array a pass walks (padded scratch rows
+  // included): every pass has at most elems / n lines of length n
+  const std::size_t elems =
+      std::max({scratch_elems(gout_, cs), scratch_elems(gin_, cs), scratch_elems(gin_, cs / 2)});
+  auto add_twiddles = [&](int n) {
+    const int n1 = long_line_split(n, fp64_), m1 = long_line_split(n, true);
+    for (int len : {n, n1, n1 > 0 ? n / n1 : 0, m1, m1 > 0 ? n / m1 : 0}) {
       if (len <= 0 || tw_.count(len)) continue;
       tw_[len] = upload_twiddles(len);
       workspace_bytes_ += 16 * static_cast<std::size_t>(len);
     }
-    if (n1 > 0) long_lines = true;
+  };
+  std::size_t long_need = 0;  // bytes of the long-line scratch
+  std::vector<int> blue_lengths;
+  for (std::size_t k = 0; k < axes_.size(); ++k) {
+    const int n = static_cast<int>(fft_extents_[k]);
+    add_twiddles(n);
+    if (long_line_split(n, fp64_) > 0) long_need = std::max(long_need, elems * cs);
+    if (!line_length_supported(n, fp64_) &&
+        std::find(blue_lengths.begin(), blue_lengths.end(), n) == blue_lengths.end())
+      blue_lengths.push_back(n);
   }
-
-  const std::size_t cs = fp64_ ? 16 : 8;
-  if (long_lines && total_size(gout_) > 0) {
-    // every line of a pass, complex and whole, over the padded scratch rows
-    const std::size_t elems =
-        std::max({scratch_elems(gout_, cs), scratch_elems(gin_, cs), scratch_elems(gin_, cs / 2)});
-    long_bytes_ = elems * cs;
+  for (int n : blue_lengths) {
+    int m = 1;
+    while (m < 2 * n - 1) m *= 2;
+    add_twiddles(m);
+    const std::size_t lines = (elems + n - 1) / n;
+    blue_bytes_ = std::max(blue_bytes_, lines * static_cast<std::size_t>(m) * cs);
+    if (long_line_split(m, fp64_) > 0) long_need = std::max(long_need, lines * static_cast<std::size_t>(m) * cs);
+    // the FP64 transform of the filter at set-up
+    if (long_line_split(m, true) > 0) long_need = std::max(long_need, static_cast<std::size_t>(m) * 16);
+  }
+  if (long_need > 0 && total_size(gout_) > 0) {
+    long_bytes_ = long_need;
     hipError_t err = hipMalloc(&long_scratch_, long_bytes_);
     if (err != hipSuccess) throw Error(DFFT_ERROR_OUT_OF_MEMORY, "hipMalloc of the long-line scratch failed");
     workspace_bytes_ += long_bytes_;
+  }
+  if (!blue_lengths.empty() && total_size(gout_) > 0) {
+    for (void*& b : blue_s_) {
+      hipError_t err = hipMalloc(&b, blue_bytes_);
+      if (err != hipSuccess) throw Error(DFFT_ERROR_OUT_OF_MEMORY, "hipMalloc of the Bluestein scratch failed");
+      workspace_bytes_ += blue_bytes_;
+    }
+    for (int n : blue_lengths) setup_bluestein(n, (elems + n - 1) / n);
   }
   if (d_.exec_mode == DFFT_EXEC_FUSED) {
     plan_fused();
@@ -511,10 +546,10 @@ void Plan::run_fused(void* const* in, void* const* out, bool forward, hipStream_
       p1.line_begin = c0 * per1;
       p1.nlines = nc * per1;
       mark(stream);
-      check_hip(launch_line_pass(p0, fp64_, stream), "line pass");
+      check_hip(launch_pass(p0, stream), "line pass");
       mark(stream);
       mark(stream);
-      check_hip(launch_line_pass(p1, fp64_, stream), "line pass");
+      check_hip(launch_pass(p1, stream), "line pass");
       mark(stream);
     }
     i0 = 2;
@@ -527,7 +562,7 @@ void Plan::run_fused(void* const* in, void* const* out, bool forward, hipStream_
     TraceRange range("dfft::%s pass %zu axis %d%s", forward ? "forward" : "backward", i, passes[i].axis,
                      first ? " (gather from the input ranks)" : last ? " (scatter to the output ranks)" : "");
     mark(stream);
-    check_hip(launch_line_pass(p, fp64_, stream), "line pass");
+    check_hip(launch_pass(p, stream), "line pass");
     mark(stream);
   }
   if (alias) {
@@ -614,9 +649,118 @@ void Plan::execute_forward_products(void* const* u3, void* const* out, hipStream
     LinePass p = make_pass(passes[i], scr, b, false, last ? scale : 1.0);
     TraceRange pr("dfft::forward pass %zu axis %d%s", i, passes[i].axis, last ? " (scatter to the output ranks)" : "");
     mark(stream);
-    check_hip(launch_line_pass(p, fp64_, stream), "line pass");
+    check_hip(launch_pass(p, stream), "line pass");
     mark(stream);
   }
+}
+
+// An (rows, m) complex array transformed along its rows (length m).
+LinePass Plan::dense_pass(int rows, int m, void* src, void* dst, bool inverse, bool fp64) const {
+  auto dense = [&](void* ptr) {
+    DDesc d{};
+    d.ndim = 2;
+    const int32_t ext[2] = {rows, m};
+    for (int a = 0; a < 2; ++a) {
+      d.ext[a] = ext[a];
+      d.ntopo[a] = 1;
+      d.base[a] = ext[a];
+      d.map[a] = a;
+      d.div_big[a] = FastDiv(static_cast<uint32_t>(ext[a] + 1));
+      d.div_small[a] = FastDiv(static_cast<uint32_t>(ext[a]));
+    }
+    d.ptr[0] = static_cast<char*>(ptr);
+    return d;
+  };
+  LinePass q{};
+  q.in = dense(src);
+  q.out = dense(dst);
+  q.axis = 1;
+  q.nbatch = 1;
+  q.batch_axis[0] = 0;
+  q.batch_ext[0] = rows;
+  q.batch_div[0] = FastDiv(static_cast<uint32_t>(rows));
+  q.nlines = rows;
+  q.kind = kC2C;
+  q.n = m;
+  q.inverse = inverse ? 1 : 0;
+  q.tw = twiddles(m);
+  q.scale = 1.0;
+  q.split_lds = 1;
+  const int m1 = long_line_split(m, fp64);
+  if (m1 > 0) {
+    const std::size_t need = static_cast<std::size_t>(rows) * m * (fp64 ? 16 : 8);
+    if (!long_scratch_ || need > long_bytes_) throw Error(DFFT_ERROR_UNSUPPORTED, "long-line scratch too small");
+    q.long_n1 = m1;
+    q.long_tw1 = twiddles(m1);
+    q.long_tw2 = twiddles(m / m1);
+    q.long_scratch = long_scratch_;
+  }
+  return q;
+}
+
+// Tables of the Bluestein transform of length n: the chirp exp(-pi i e / n)
+// (e = j^2 mod 2n, evaluated in long double, rounded once to FP64) and the
+// FP64 m-point transform of the filter b_j = conj(chirp(j^2)), |j| < n
+// (b at m - j for negative j), divided by m.
+void Plan::setup_bluestein(int n, std::size_t max_lines) {
+  Bluestein b;
+  b.m = 1;
+  while (b.m < 2 * n - 1) {
+    b.m *= 2;
+    ++b.log2m;
+  }
+  if (max_lines * static_cast<std::size_t>(b.m) >= (std::size_t(1) << 31))
+    throw Error(DFFT_ERROR_UNSUPPORTED, "too many Bluestein lines of length " + std::to_string(n));
+  std::vector<double> chirp(4 * static_cast<std::size_t>(n));
+  for (int e = 0; e < 2 * n; ++e) {
+    const long double a = 3.141592653589793238462643383279502884L * e / n;
+    chirp[2 * e] = static_cast<double>(cosl(a));
+    chirp[2 * e + 1] = static_cast<double>(-sinl(a));
+  }
+  std::vector<double> filt(2 * static_cast<std::size_t>(b.m), 0.0);
+  for (int64_t j = 0; j < n; ++j) {
+    const int64_t e = (j * j) % (2 * static_cast<int64_t>(n));
+    const double re = chirp[2 * e], im = -chirp[2 * e + 1];
+    filt[2 * j] = re;
+    filt[2 * j + 1] = im;
+    if (j > 0) {
+      filt[2 * (b.m - j)] = re;
+      filt[2 * (b.m - j) + 1] = im;
+    }
+  }
+  const std::size_t cb = chirp.size() * sizeof(double), fb = filt.size() * sizeof(double);
+  void* tmp = nullptr;
+  check_hip(hipMalloc(&b.chirp, cb), "hipMalloc of the Bluestein chirp");
+  blue_[n] = b;  // released with the plan from here on
+  check_hip(hipMalloc(&blue_[n].filt, fb), "hipMalloc of the Bluestein filter");
+  check_hip(hipMalloc(&tmp, fb), "hipMalloc");
+  hipError_t err = hipMemcpy(b.chirp, chirp.data(), cb, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(tmp, filt.data(), fb, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    LinePass q = dense_pass(1, b.m, tmp, blue_[n].filt, false, true);
+    q.scale = 1.0 / b.m;
+    err = launch_line_pass(q, true, nullptr);
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(nullptr);
+  (void)hipFree(tmp);
+  check_hip(err, "Bluestein filter transform");
+  workspace_bytes_ += cb + fb;
+}
+
+hipError_t Plan::launch_pass(const LinePass& p, hipStream_t s) {
+  auto it = blue_.find(p.n);
+  if (it == blue_.end()) return launch_line_pass(p, fp64_, s);
+  if (p.nlines <= 0) return hipSuccess;
+  const Bluestein& b = it->second;
+  const std::size_t need = static_cast<std::size_t>(p.nlines) * b.m * (fp64_ ? 16 : 8);
+  if (need > blue_bytes_) return hipErrorInvalidValue;
+  hipError_t e = launch_bluestein_pre(p, b.log2m, b.chirp, blue_s_[0], fp64_, s);
+  if (e == hipSuccess) e = launch_line_pass(dense_pass(p.nlines, b.m, blue_s_[0], blue_s_[1], false, fp64_), fp64_, s);
+  if (e == hipSuccess)
+    e = launch_bluestein_mul(static_cast<int64_t>(p.nlines) << b.log2m, b.log2m, b.filt, blue_s_[1], fp64_, s);
+  if (e == hipSuccess) e = launch_line_pass(dense_pass(p.nlines, b.m, blue_s_[1], blue_s_[0], true, fp64_), fp64_, s);
+  if (e == hipSuccess) e = launch_bluestein_post(p, b.log2m, b.chirp, blue_s_[0], fp64_, s);
+  return e;
 }
 
 void Plan::execute(void* const* in, void* const* out, int direction, hipStream_t stream) {
@@ -817,12 +961,12 @@ void Plan::autotune(void* const* in, void* const* out, hipStream_t stream) {
       // sample does not decide)
       auto time_spec = [&](const PassSpec& s) -> float {
         LinePass p = make_pass(s, a, b, !forward, 1.0);
-        if (launch_line_pass(p, fp64_, stream) != hipSuccess) {
+        if (launch_pass(p, stream) != hipSuccess) {
           (void)hipGetLastError();
           return -1.f;
         }
         check_hip(hipEventRecord(e0, stream), "hipEventRecord");
-        for (int k = 0; k < 3; ++k) check_hip(launch_line_pass(p, fp64_, stream), "line pass");
+        for (int k = 0; k < 3; ++k) check_hip(launch_pass(p, stream), "line pass");
         check_hip(hipEventRecord(e1, stream), "hipEventRecord");
         check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
         float ms = 0.f;
@@ -1009,7 +1153,7 @@ void Plan::run_blockwalk(void* const* in, void* const* out, bool forward, hipStr
       scaled = true;
       TraceRange range("dfft::FFT block %zu axis %zu", static_cast<std::size_t>(b.block_idx), a);
       mark(stream);
-      check_hip(launch_line_pass(p, fp64_, stream), "line pass");
+      check_hip(launch_pass(p, stream), "line pass");
       mark(stream);
       st = dst;
     }

@@ -122,6 +122,20 @@ class Plan {
   // transform (nlines * n values of a pass)
   void* long_scratch_ = nullptr;
   std::size_t long_bytes_ = 0;
+  // lengths with a prime factor beyond one LDS line: Bluestein through two
+  // m-point transforms of an (nlines, m) scratch pair
+  struct Bluestein {
+    int m = 0, log2m = 0;
+    void* chirp = nullptr;  // exp(-pi i e / n), e in [0, 2n), FP64
+    void* filt = nullptr;   // m-point transform of the chirp filter / m, FP64
+  };
+  std::map<int, Bluestein> blue_;
+  void* blue_s_[2] = {nullptr, nullptr};
+  std::size_t blue_bytes_ = 0;
+  void setup_bluestein(int n, std::size_t max_lines);
+  LinePass dense_pass(int rows, int m, void* src, void* dst, bool inverse, bool fp64) const;
+  // a line pass through the kernel its length needs (Bluestein included)
+  hipError_t launch_pass(const LinePass& p, hipStream_t s);
   std::vector<void*> walk_bufs_[2];  // block-walk ping-pong local arrays per rank
   std::unique_ptr<VendorFFT> vendor_;  // DFFT_EXEC_VENDOR: rocFFT over the gathered global array
   std::size_t workspace_bytes_ = 0;

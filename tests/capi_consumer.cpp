@@ -108,7 +108,7 @@ int host() {
   cases.push_back({"brick", desc3({8, 8, 8}, {8, 8, 8}, {2, 2, 2}, {2, 2, 2}, DFFT_C2C, DFFT_FP64, 8)});
   cases.push_back({"empty", desc3({8, 8, 8}, {8, 8, 8}, {0, 1, 1}, {0, 1, 1}, DFFT_C2C, DFFT_FP64, 0)});
   cases.push_back({"slab_to_pencil", desc3({8, 8, 8}, {8, 8, 8}, {4, 1, 1}, {1, 2, 2}, DFFT_C2C, DFFT_FP64, 4)});
-  cases.push_back({"length_without_kernel", desc3({6007, 4, 4}, {6007, 4, 4}, {1, 2, 1}, {1, 1, 2}, DFFT_C2C, DFFT_FP64, 2)});
+  cases.push_back({"extent_too_large", desc3({1u << 30, 4, 4}, {1u << 30, 4, 4}, {1, 2, 1}, {1, 1, 2}, DFFT_C2C, DFFT_FP64, 2)});
   {
     dfft_desc d = desc3({8, 7, 5}, {8, 7, 5}, {2, 1, 1}, {1, 2, 1}, DFFT_C2C, DFFT_FP64, 2);
     d.axes[2] = 1;  // repeated axis

@@ -66,7 +66,7 @@ def test_invalid_plans_return_reference_status_codes():
         "pencil_rank2": dfft.ERROR_UNSUPPORTED,                   # Plan.hpp:101-103
         "fft_rank_4": dfft.ERROR_INVALID_ARGUMENT,                # Plan.hpp:137 (static_assert)
         "bad_norm": dfft.ERROR_INVALID_ARGUMENT,
-        "length_without_kernel": dfft.ERROR_UNSUPPORTED,          # no counterpart: no line kernel
+        "extent_too_large": dfft.ERROR_UNSUPPORTED,               # no counterpart: int32 element indices
     }
     for k, v in expect.items():
         assert got[k] == v, (k, got[k])

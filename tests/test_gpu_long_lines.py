@@ -49,23 +49,46 @@ def test_long_line_closed_form(dev):
     assert np.abs(got - want).max() <= 1e-9 * n * 4
 
 
-def test_prime_beyond_one_line_is_refused(dev):
-    import torch
-    import dfft
-    n = 6007  # prime, 2 * 6007 complex128 values exceed the LDS of one workgroup
-    ins = [torch.zeros(n, 2, dtype=torch.complex128, device=dev) for _ in range(2)]
-    outs = [torch.zeros(n // 2 + (1 - r), 4, dtype=torch.complex128, device=dev) for r in range(2)]
-    with pytest.raises(dfft.DFFTError) as e:
-        dfft.Plan(ins, outs, [0, 1], [1, 2], [2, 1], global_in_extents=[n, 4], global_out_extents=[n, 4])
-    assert e.value.code == dfft.ERROR_UNSUPPORTED
+BLUESTEIN_CASES = [
+    ([6007, 4], [0, 1], [1, 2], [2, 1], True),         # 6007 prime: Bluestein, m = 16384 = 128 x 128
+    ([3, 4, 6007], [0, 1, 2], [3, 1, 1], [1, 3, 1], True),  # ... as the R2C axis
+    ([10243, 2], [0, 1], [1, 2], [2, 1], False),       # FP32 beyond 10240, prime
+]
 
 
+@pytest.mark.parametrize("gin,axes,tin,tout,fp64", BLUESTEIN_CASES)
 @pytest.mark.parametrize("real", [False, True])
-def test_long_line_adjoint_dot_product(dev, real):
-    """<F x, y> = <x, F^* y> with a 8192-point axis (R2C axis when real)."""
+@pytest.mark.parametrize("exec_mode", [0, 1])
+def test_prime_lengths_beyond_one_line_run_bluestein(dev, gin, axes, tin, tout, fp64, real, exec_mode):
+    x = seeded_input(gin, real, 5)
+    outs, backs, ins, _ = run_gpu_plan(dev, x, gin, axes, tin, tout, real, fp64, "ortho", exec_mode)
+    ref = reference_spectrum(x, gin, axes, tout, real, "ortho")
+    tol = 1e-12 if fp64 else 1e-5
+    e_fwd, e_rt = gather_rel_l2(outs, ref), gather_rel_l2(backs, ins)
+    assert e_fwd <= tol, f"forward rel-L2 {e_fwd:.3e}"
+    assert e_rt <= tol, f"round trip rel-L2 {e_rt:.3e}"
+
+
+def test_bluestein_single_mode(dev):
+    """A single mode along a 6007-point (prime) axis: n at k = m, 0 elsewhere."""
+    n, m = 6007, 2999
+    j = np.arange(n)
+    x = np.broadcast_to(np.exp(2j * np.pi * m * j / n)[:, None], (n, 2)).copy()
+    outs, _, _, _ = run_gpu_plan(dev, x, [n, 2], [0, 1], [1, 2], [2, 1], False, True)
+    got = PE.gather(outs, out_extents([n, 2], [0, 1], False), [2, 1], np.complex128)
+    want = np.zeros((n, 2), np.complex128)
+    want[m, 0] = n * 2
+    assert np.abs(got - want).max() <= 1e-9 * n * 2
+
+
+@pytest.mark.parametrize("gin", [[4, 6, 8192], [4, 6, 6007]])
+@pytest.mark.parametrize("real", [False, True])
+def test_long_line_adjoint_dot_product(dev, gin, real):
+    """<F x, y> = <x, F^* y> with a 8192-point axis (two launches) or a
+    6007-point one (Bluestein), the R2C axis when real."""
     import torch
     import dfft
-    gin, axes, tin, tout = [4, 6, 8192], [0, 1, 2], [2, 1, 1], [1, 2, 1]
+    axes, tin, tout = [0, 1, 2], [2, 1, 1], [1, 2, 1]
     gout = out_extents(gin, axes, real)
     cdt = torch.complex128
     idt = torch.float64 if real else cdt
