@@ -1387,12 +1387,12 @@ hipError_t launch_products(LinePass p, hipStream_t s) {
   if (!p.products || p.kind != kR2C || p.inverse) return hipErrorInvalidValue;
   const bool split = p.split_lds != 0;
   const size_t line_bytes = split ? line_lds_bytes<Real, N, 8, true>() : line_lds_bytes<Real, N, 8, false>();
-  // pairs per block: whole positions (3 pairs each), near the pass's own
-  // choice (tuned for its values per thread, which may differ from 8), cut
-  // to what this shape can hold
-  int B = p.lines_per_block;
-  if (B <= 0) B = p.lines_fast ? 8 : (T >= 256 ? 1 : 256 / T);
-  B = B >= 3 ? B / 3 * 3 : 3;
+  // pairs per block: whole positions (3 pairs each) filling at least three
+  // full waves (192 lanes; the 256^3 NS step took 8.01 ms with it, 8.23 ms
+  // with the R2C pass's own 3 pairs of 32 lanes, 8.36-8.49 ms with 6 or 12
+  // waves), cut to what this shape can hold
+  int B = 3;
+  while (B * T < 192 || (B * T) % 64) B += 3;
   while (B > 3 && (B * T > 1024 || static_cast<size_t>(B) * line_bytes > 160 * 1024)) B -= 3;
   const size_t smem = static_cast<size_t>(B) * line_bytes;
   if (B * T > 1024 || smem > 160 * 1024 || p.nlines % 6 != 0 || p.line_begin % 6 != 0)
