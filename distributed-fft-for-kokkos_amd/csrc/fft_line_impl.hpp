@@ -634,7 +634,7 @@ template <typename Real, int N, int RM, int S> struct StageLoop2 {
 // line pairs are loaded as two 8-byte elements, and the second element of a
 // pair whose second line is padding (the next row's first element there, or
 // past the end of the rank's array) is not loaded.
-template <typename Real, int N, int RM, bool NT, bool SPLIT_IN = false>
+template <typename Real, int N, int RM, bool NT, int SPLIT_IN = 0>
 __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, int B, unsigned char* lds) {
   using C = cplx_t<Real>;
   using Cfg = FastCfg<Real, N, RM>;
@@ -656,7 +656,13 @@ __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, 
   if (valid) io = make_line_io<Real, NT>(p, la);
 
   bool second = false;
-  if constexpr (SPLIT_IN) second = valid && !padding_line<Real>(p, la + 1);
+  if constexpr (SPLIT_IN == 1) second = valid && !padding_line<Real>(p, la + 1);
+  // SPLIT_IN == 2: line la + 1 has addresses of its own on both sides
+  [[maybe_unused]] LineIO<Real, NT> io2 = io;
+  if constexpr (SPLIT_IN == 2) {
+    second = valid && la + 1 < p.nlines && !padding_line<Real>(p, la + 1);
+    if (second) io2 = make_line_io<Real, NT>(p, la + 1);
+  }
   C va[R], vb[R];
   constexpr int r0 = Cfg::P.rad[0];
   constexpr int NB0 = R / r0;
@@ -664,10 +670,20 @@ __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, 
   for (int q = 0; q < NB0; ++q) {
     const C* src = reinterpret_cast<const C*>(io.ib) + static_cast<int64_t>(t * NB0 + q) * io.is;
     const int64_t step = static_cast<int64_t>(N / r0) * io.is;
+    [[maybe_unused]] const C* src2 = nullptr;
+    [[maybe_unused]] int64_t step2 = 0;
+    if constexpr (SPLIT_IN == 2) {
+      src2 = reinterpret_cast<const C*>(io2.ib) + static_cast<int64_t>(t * NB0 + q) * io2.is;
+      step2 = static_cast<int64_t>(N / r0) * io2.is;
+    }
 #pragma unroll
     for (int j = 0; j < r0; ++j) {
       C a, c;
-      if constexpr (SPLIT_IN) {
+      if constexpr (SPLIT_IN == 2) {
+        a = valid ? gload_c<NT, Real>(src) : C{Real(0), Real(0)};
+        c = second ? gload_c<NT, Real>(src2) : C{Real(0), Real(0)};
+        src2 += step2;
+      } else if constexpr (SPLIT_IN == 1) {
         a = valid ? gload_c<NT, Real>(src) : C{Real(0), Real(0)};
         c = second ? gload_c<NT, Real>(src + 1) : C{Real(0), Real(0)};
       } else {
@@ -688,7 +704,14 @@ __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, 
         va, vb, ba, bb, [t](int q, int j) { return t * NBl + q + j * (N / rl); },
         [t](int, int j) { return t + j * T; });
   }
-  if (valid) {
+  // SPLIT_IN == 2: the pair is stored as one 16-byte element pair where the
+  // second line's output follows the first's on a 16-byte boundary at every
+  // position (even element stride), else as two elements
+  bool pair_out = true;
+  if constexpr (SPLIT_IN == 2)
+    pair_out = second && io2.ob == io.ob + sizeof(C) && io2.os == io.os && (io.os & 1) == 0 &&
+               (reinterpret_cast<uintptr_t>(io.ob) & 15) == 0;
+  if (valid && pair_out) {
     C* dst = reinterpret_cast<C*>(io.ob) + static_cast<int64_t>(t) * io.os;
     const int64_t step = static_cast<int64_t>(T) * io.os;
 #pragma unroll
@@ -702,6 +725,28 @@ __device__ __forceinline__ void fast_line2_body(const LinePass& p, int32_t blk, 
       x.w = c.y * io.scale;
       gstore<NT>(reinterpret_cast<V4*>(dst), x);
       dst += step;
+    }
+  } else if constexpr (SPLIT_IN == 2) {
+    if (valid) {
+      C* dst = reinterpret_cast<C*>(io.ob) + static_cast<int64_t>(t) * io.os;
+      const int64_t step = static_cast<int64_t>(T) * io.os;
+      [[maybe_unused]] C* dst2 = nullptr;
+      [[maybe_unused]] int64_t stepb = 0;
+      if (second) {
+        dst2 = reinterpret_cast<C*>(io2.ob) + static_cast<int64_t>(t) * io2.os;
+        stepb = static_cast<int64_t>(T) * io2.os;
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        C a = p.inverse ? cconj(va[j]) : va[j];
+        gstore_c<NT, Real>(dst, C{a.x * io.scale, a.y * io.scale});
+        dst += step;
+        if (second) {
+          C c = p.inverse ? cconj(vb[j]) : vb[j];
+          gstore_c<NT, Real>(dst2, C{c.x * io.scale, c.y * io.scale});
+          dst2 += stepb;
+        }
+      }
     }
   }
 }
@@ -1143,7 +1188,7 @@ __global__ void __launch_bounds__(1024) fft_line_fast(const LinePass p) {
   fast_line_body<Real, N, RM, NT, SPLIT>(p, blk, p.lines_per_block, smem_raw);
 }
 
-template <typename Real, int N, int RM, bool NT, bool SPLIT_IN>
+template <typename Real, int N, int RM, bool NT, int SPLIT_IN>
 __global__ void __launch_bounds__(1024) fft_line2_fast(const LinePass p) {
   extern __shared__ unsigned char smem_raw[];
   const int32_t blk = pass_block(p, blockIdx.x, gridDim.x);
@@ -1267,6 +1312,35 @@ inline int dual_lines_mode(const LinePass& p, int B) {
 
 inline bool dual_lines_apply_impl(const LinePass& p, int B) { return dual_lines_mode(p, B) == 0; }
 
+// Where dual_lines_mode refuses odd output rows: an unpadded output whose
+// rows run along the fastest batch axis, both sides with that axis innermost
+// and undistributed.  Enumerated over the unpadded rows, lines 2b and 2b+1
+// are consecutive output elements, also across a row end; the kernel then
+// takes each line's addresses on their own (two-lines mode 2) and pairs the
+// stores where they are adjacent and 16-byte aligned.  Returns false or
+// rewrites q into that enumeration.
+inline bool dual_lines_linear(const LinePass& p, int B, LinePass& q) {
+  if (!p.dual_lines || p.kind != kC2C || !p.lines_fast || p.nbatch < 1 || B < 2 || B % 2) return false;
+  if (p.line_begin != 0) return false;
+  const int j = p.nbatch - 1, fa = p.batch_axis[j];
+  for (const DDesc* d : {&p.in, &p.out}) {
+    if (d->layout_left) return false;
+    if (d->map[d->ndim - 1] != fa || d->ntopo[fa] != 1) return false;
+  }
+  if (p.out.pitch > 0 || p.out.ext[fa] % 2 == 0) return false;
+  q = p;
+  if (p.pad_real > 0) {
+    if (p.pad_axis != fa || p.batch_ext[j] <= 0) return false;
+    q.nlines = p.nlines / p.batch_ext[j] * p.pad_real;
+    q.batch_ext[j] = p.pad_real;
+    q.batch_div[j] = FastDiv(static_cast<uint32_t>(p.pad_real));
+    q.pad_real = 0;
+    q.pad_axis = 0;
+  }
+  q.dual_split = 2;
+  return true;
+}
+
 template <typename Real, int N, int RM>
 hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
   if constexpr (!dual_kernel_exists<Real, N>()) {
@@ -1278,15 +1352,19 @@ hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
     const size_t smem = static_cast<size_t>(B) * dual_line_bytes<Real, N, RM>();
     if (P2 * T > 1024 || smem > 160 * 1024) return hipErrorInvalidConfiguration;
     p.lines_per_block = B;
-    static std::atomic<size_t> lds_limit[4] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
+    static std::atomic<size_t> lds_limit[6] = {64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024, 64 * 1024};
     const int nt = p.nontemporal ? 1 : 0;
-    const bool split = (p.dual_split & 1) != 0;
-    const void* fn =
-        nt ? (split ? reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true, true>)
-                    : reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true, false>))
-           : (split ? reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false, true>)
-                    : reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false, false>));
-    const hipError_t e = raise_lds_limit(lds_limit[2 * (split ? 1 : 0) + nt], fn, smem);
+    // 0: paired 16-byte loads, 1: adjacent input lines as two 8-byte loads,
+    // 2: every line with its own addresses
+    const int mode = p.dual_split & 2 ? 2 : (p.dual_split & 1);
+    const void* fns[2][3] = {
+        {reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false, 0>),
+         reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false, 1>),
+         reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, false, 2>)},
+        {reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true, 0>),
+         reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true, 1>),
+         reinterpret_cast<const void*>(&fft_line2_fast<Real, N, RM, true, 2>)}};
+    const hipError_t e = raise_lds_limit(lds_limit[3 * nt + mode], fns[nt][mode], smem);
     if (e != hipSuccess) return e;
     const unsigned grid = static_cast<unsigned>((static_cast<int64_t>(p.nlines) + B - 1) / B);
     if (grid == 0) return hipSuccess;
@@ -1295,12 +1373,15 @@ hipError_t launch_dual(LinePass p, int B, hipStream_t s) {
       const int32_t C = (inner > 0 && inner % B == 0) ? inner / B : 0;
       p.swizzle = (C > 1 && grid % static_cast<unsigned>(C) == 0 && grid / C > 1) ? C : 0;
     }
+    const dim3 g(grid), b(P2 * T);
     if (nt) {
-      if (split) hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true, true>), dim3(grid), dim3(P2 * T), smem, s, p);
-      else hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true, false>), dim3(grid), dim3(P2 * T), smem, s, p);
+      if (mode == 2) hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true, 2>), g, b, smem, s, p);
+      else if (mode == 1) hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true, 1>), g, b, smem, s, p);
+      else hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, true, 0>), g, b, smem, s, p);
     } else {
-      if (split) hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false, true>), dim3(grid), dim3(P2 * T), smem, s, p);
-      else hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false, false>), dim3(grid), dim3(P2 * T), smem, s, p);
+      if (mode == 2) hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false, 2>), g, b, smem, s, p);
+      else if (mode == 1) hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false, 1>), g, b, smem, s, p);
+      else hipLaunchKernelGGL((fft_line2_fast<Real, N, RM, false, 0>), g, b, smem, s, p);
     }
     return hipGetLastError();
   }
@@ -1319,6 +1400,8 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
       p.dual_split = mode;
       return launch_dual<Real, N, RM>(p, Bd, s);
     }
+    LinePass q;
+    if (dual_lines_linear(p, Bd, q)) return launch_dual<Real, N, RM>(q, Bd, s);
   }
   const bool split = split_available<Real, N>() && p.split_lds != 0;
   const size_t line_bytes = split ? line_lds_bytes<Real, N, RM, true>() : line_lds_bytes<Real, N, RM, false>();
