@@ -36,6 +36,22 @@ def test_long_lines_match_oracle(dev, gin, axes, tin, tout, fp64, real, exec_mod
     assert e_rt <= tol, f"round trip rel-L2 {e_rt:.3e}"
 
 
+@pytest.mark.parametrize("gin,axes,tin,tout", [
+    ([8192, 6, 4], [0, 1, 2], [1, 2, 1], [1, 1, 2]),   # two launches
+    ([6007, 4, 3], [0, 1, 2], [1, 2, 1], [1, 1, 2]),   # Bluestein
+])
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("exec_mode", [0, 1])
+def test_long_lines_layout_left_views(dev, gin, axes, tin, tout, real, exec_mode):
+    """The same paths on LayoutLeft (column-major) views."""
+    x = seeded_input(gin, real, 9)
+    outs, backs, ins, _ = run_gpu_plan(dev, x, gin, axes, tin, tout, real, True, "backward", exec_mode,
+                                       view_layout="left")
+    ref = reference_spectrum(x, gin, axes, tout, real)
+    assert gather_rel_l2(outs, ref) <= 1e-12
+    assert gather_rel_l2(backs, ins) <= 1e-12
+
+
 def test_long_line_closed_form(dev):
     """A single mode exp(2 pi i m j / n) along an 8192-point axis transforms
     to n at k = m and 0 elsewhere (independent of the oracle's FFT)."""
