@@ -441,7 +441,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the P virtual ranks share one MI355X; runs on more "
-              "than one GPU are not part of this build", file=sys.stderr)
+              "than one GPU are not measured by this benchmark", file=sys.stderr)
         sys.exit(2)
 
     import torch
