@@ -52,6 +52,29 @@ def test_long_lines_layout_left_views(dev, gin, axes, tin, tout, real, exec_mode
     assert gather_rel_l2(backs, ins) <= 1e-12
 
 
+@pytest.mark.parametrize("gin,tin,tout", [
+    ([6, 8192, 4], [2, 1, 1], [1, 1, 2]),   # the long axis in the first two (chunked) passes
+    ([6, 6007, 4], [2, 1, 1], [1, 1, 2]),   # Bluestein in a chunked pass
+])
+@pytest.mark.parametrize("configure", ["chunk", "autotune"])
+def test_long_lines_chunked_and_tuned(dev, gin, tin, tout, configure):
+    """Chunked first passes (lines of one chunk at a line offset) and the
+    autotuner on plans whose passes run two launches or Bluestein."""
+    def conf(plan, ins, outs):
+        if configure == "chunk":
+            plan.set_option("chunk", 2)
+        else:
+            plan.autotune(ins, outs)
+
+    for real in (False, True):
+        x = seeded_input(gin, real, 13)
+        outs, backs, ins, _ = run_gpu_plan(dev, x, gin, [0, 1, 2], tin, tout, real, True, "backward", 0,
+                                           configure=conf)
+        ref = reference_spectrum(x, gin, [0, 1, 2], tout, real)
+        assert gather_rel_l2(outs, ref) <= 1e-12
+        assert gather_rel_l2(backs, ins) <= 1e-12
+
+
 def test_long_line_closed_form(dev):
     """A single mode exp(2 pi i m j / n) along an 8192-point axis transforms
     to n at k = m and 0 elsewhere (independent of the oracle's FFT)."""
