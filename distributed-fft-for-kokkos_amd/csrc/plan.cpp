@@ -729,20 +729,24 @@ void Plan::setup_bluestein(int n, std::size_t max_lines) {
     }
   }
   const std::size_t cb = chirp.size() * sizeof(double), fb = filt.size() * sizeof(double);
-  void* tmp = nullptr;
   check_hip(hipMalloc(&b.chirp, cb), "hipMalloc of the Bluestein chirp");
   blue_[n] = b;  // released with the plan from here on
   check_hip(hipMalloc(&blue_[n].filt, fb), "hipMalloc of the Bluestein filter");
-  check_hip(hipMalloc(&tmp, fb), "hipMalloc");
+  struct Tmp {
+    void* p = nullptr;
+    ~Tmp() {
+      if (p) (void)hipFree(p);
+    }
+  } tmp;
+  check_hip(hipMalloc(&tmp.p, fb), "hipMalloc");
   hipError_t err = hipMemcpy(b.chirp, chirp.data(), cb, hipMemcpyHostToDevice);
-  if (err == hipSuccess) err = hipMemcpy(tmp, filt.data(), fb, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(tmp.p, filt.data(), fb, hipMemcpyHostToDevice);
   if (err == hipSuccess) {
-    LinePass q = dense_pass(1, b.m, tmp, blue_[n].filt, false, true);
+    LinePass q = dense_pass(1, b.m, tmp.p, blue_[n].filt, false, true);
     q.scale = 1.0 / b.m;
     err = launch_line_pass(q, true, nullptr);
   }
   if (err == hipSuccess) err = hipStreamSynchronize(nullptr);
-  (void)hipFree(tmp);
   check_hip(err, "Bluestein filter transform");
   workspace_bytes_ += cb + fb;
 }
