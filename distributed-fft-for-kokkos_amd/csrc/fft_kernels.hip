@@ -625,6 +625,38 @@ bool line_length_supported(int n, bool fp64) {
   return line_length_single_pass(n, fp64) || long_line_split(n, fp64) > 0;
 }
 
+// Cost model (ms per 16.8M complex elements, forward + backward, measured on
+// (n, W) arrays with prime n from 11 to 4099): the generic kernel costs
+// 0.5 + 0.031 * (sum of its radices) -- its r-term sums make a prime p cost
+// ~p operations per element -- in both precisions; Bluestein costs
+// c * m / n + 0.3 with c = 1.15 (FP64) / 0.7 (FP32) while the m-point passes
+// have a fast kernel (m <= 2048) and c = 10.9 / 12 beyond.  Measured pairs
+// (generic / Bluestein, FP64): 61: 2.05 / 2.21, 127: 3.84 / 2.64, 1009: 34.8
+// / 2.6, 2053: 94 / 21.8, 4099: 376 / 21.8 ms; FP32 31: 1.09 / 1.39, 61:
+// 1.87 / 1.41 ms.  Bluestein is taken where it models at least 20 % cheaper
+// than every other kernel that can run the length.
+bool bluestein_preferred(int n, bool fp64) {
+  if (n < 2 || has_fast_line_kernel(n)) return false;
+  if (!line_length_supported(n, fp64)) return true;
+  auto generic = [](int len) {
+    GenPlan g;
+    if (!make_gen_plan(len, g)) return 1e30;
+    int sum = 0;
+    for (int s = 0; s < g.nst; ++s) sum += g.rad[s];
+    return 0.5 + 0.031 * sum;
+  };
+  auto pass_cost = [&](int len) { return has_fast_line_kernel(len) ? 0.5 : generic(len); };
+  double other;
+  const int n1 = long_line_split(n, fp64);
+  if (n1 > 0) other = pass_cost(n1) + pass_cost(n / n1) + 0.5;  // two launches and their twiddle
+  else other = generic(n);
+  int64_t m = 1;
+  while (m < 2 * static_cast<int64_t>(n) - 1) m *= 2;
+  const double c = m <= 2048 ? (fp64 ? 1.15 : 0.7) : (fp64 ? 10.9 : 12.0);
+  const double blue = c * static_cast<double>(m) / n + 0.3;
+  return blue < 0.8 * other;
+}
+
 bool products_pass_supported(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
 
 hipError_t launch_products_pass(const LinePass& p, hipStream_t s) {

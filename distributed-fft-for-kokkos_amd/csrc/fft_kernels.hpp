@@ -125,6 +125,10 @@ bool line_length_single_pass(int n, bool fp64);
 int long_line_split(int n, bool fp64);
 // single pass or a two-pass split
 bool line_length_supported(int n, bool fp64);
+// true where the plan runs length n by Bluestein's algorithm: no other kernel
+// runs it, or a measured cost model puts Bluestein well below the generic
+// kernel (large prime factors)
+bool bluestein_preferred(int n, bool fp64);
 hipError_t launch_line_pass(const LinePass& p, bool fp64, hipStream_t s);
 // Bluestein steps of pass p over an (nlines, 2^log2m) complex scratch (see
 // fft_kernels.hip); chirp: exp(-pi i e / n), e in [0, 2n), FP64; filt: the

@@ -202,8 +202,8 @@ void Plan::allocate() {
   for (std::size_t k = 0; k < axes_.size(); ++k) {
     const int n = static_cast<int>(fft_extents_[k]);
     add_twiddles(n);
-    if (long_line_split(n, fp64_) > 0) long_need = std::max(long_need, elems * cs);
-    if (!line_length_supported(n, fp64_) &&
+    if (long_line_split(n, fp64_) > 0 && !bluestein_preferred(n, fp64_)) long_need = std::max(long_need, elems * cs);
+    if (bluestein_preferred(n, fp64_) &&
         std::find(blue_lengths.begin(), blue_lengths.end(), n) == blue_lengths.end())
       blue_lengths.push_back(n);
   }
@@ -477,7 +477,7 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   }
   if (nl >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many lines in one pass");
   p.nlines = static_cast<int32_t>(nl);
-  const int n1 = long_line_split(s.n, fp64_);
+  const int n1 = blue_.count(s.n) ? 0 : long_line_split(s.n, fp64_);
   if (n1 > 0) {
     const std::size_t need = static_cast<std::size_t>(nl) * static_cast<std::size_t>(s.n) * (fp64_ ? 16 : 8);
     if (!long_scratch_ || need > long_bytes_)

@@ -108,6 +108,25 @@ def test_prime_lengths_beyond_one_line_run_bluestein(dev, gin, axes, tin, tout, 
     assert e_rt <= tol, f"round trip rel-L2 {e_rt:.3e}"
 
 
+@pytest.mark.parametrize("gin,fp64", [
+    ([127, 6, 4], True), ([1009, 4, 3], True), ([2018, 3, 2], True),   # large prime factors: Bluestein
+    ([61, 8, 4], False), ([1009, 4, 3], False),
+    ([61, 8, 4], True), ([31, 8, 4], False),                         # small ones: the generic kernel
+])
+@pytest.mark.parametrize("real", [False, True])
+def test_lengths_with_large_prime_factors(dev, gin, fp64, real):
+    """Lengths whose largest prime factor makes the generic kernel's r-term
+    sums expensive run by Bluestein (cost model); either way the results
+    match the oracle."""
+    tin, tout = [1, 2, 1], [1, 1, 2]
+    x = seeded_input(gin, real, 21)
+    outs, backs, ins, _ = run_gpu_plan(dev, x, gin, [0, 1, 2], tin, tout, real, fp64)
+    ref = reference_spectrum(x, gin, [0, 1, 2], tout, real)
+    tol = 1e-12 if fp64 else 1e-5
+    assert gather_rel_l2(outs, ref) <= tol
+    assert gather_rel_l2(backs, ins) <= tol
+
+
 def test_bluestein_single_mode(dev):
     """A single mode along a 6007-point (prime) axis: n at k = m, 0 elsewhere."""
     n, m = 6007, 2999
