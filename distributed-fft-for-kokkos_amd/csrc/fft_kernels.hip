@@ -489,6 +489,7 @@ template <typename Real>
 hipError_t dispatch(const LinePass& p, hipStream_t s) {
   const bool fp64 = std::is_same<Real, double>::value;
   if (p.n <= 160) return fp64 ? dispatch_fast_f64_short(p, s) : dispatch_fast_f32_short(p, s);
+  if (p.n >= 4096) return fp64 ? dispatch_fast_f64_xl(p, s) : dispatch_fast_f32_xl(p, s);
   return fp64 ? dispatch_fast_f64_long(p, s) : dispatch_fast_f32_long(p, s);
 }
 
@@ -599,7 +600,7 @@ hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, 
 bool has_fast_line_kernel(int n) {
   switch (n) {
     case 2: case 4: case 8: case 16: case 32: case 64: case 96: case 128: case 160: case 256:
-    case 512: case 1024: case 2048:
+    case 512: case 1024: case 2048: case 4096: case 8192:
       return true;
     default:
       return false;
@@ -629,12 +630,13 @@ bool line_length_supported(int n, bool fp64) {
 // (n, W) arrays with prime n from 11 to 4099): the generic kernel costs
 // 0.5 + 0.031 * (sum of its radices) -- its r-term sums make a prime p cost
 // ~p operations per element -- in both precisions; Bluestein costs
-// c * m / n + 0.3 with c = 1.15 (FP64) / 0.7 (FP32) while the m-point passes
-// have a fast kernel (m <= 2048) and c = 10.9 / 12 beyond.  Measured pairs
-// (generic / Bluestein, FP64): 61: 2.05 / 2.21, 127: 3.84 / 2.64, 1009: 34.8
-// / 2.6, 2053: 94 / 21.8, 4099: 376 / 21.8 ms; FP32 31: 1.09 / 1.39, 61:
-// 1.87 / 1.41 ms.  Bluestein is taken where it models at least 20 % cheaper
-// than every other kernel that can run the length.
+// c * m / n + 0.3 with c = 1.15 (FP64) / 0.7 (FP32) for m <= 2048, 2.7 / 1.5
+// for m = 4096 and 8192 (one-line kernels of 256 and 512 lanes) and 10.9 /
+// 12 beyond (two launches of the generic kernel).  Measured pairs (generic /
+// Bluestein, FP64): 61: 2.05 / 2.21, 127: 3.84 / 2.64, 1009: 34.8 / 2.6,
+// 2053: 94 / 5.7, 4099: 376 / 21.7 ms; FP32 31: 1.09 / 1.39, 61: 1.87 / 1.41,
+// 2053: 79 / 3.4 ms.  Bluestein is taken where it models at least 20 %
+// cheaper than every other kernel that can run the length.
 bool bluestein_preferred(int n, bool fp64) {
   if (n < 2 || has_fast_line_kernel(n)) return false;
   if (!line_length_supported(n, fp64)) return true;
@@ -652,7 +654,7 @@ bool bluestein_preferred(int n, bool fp64) {
   else other = generic(n);
   int64_t m = 1;
   while (m < 2 * static_cast<int64_t>(n) - 1) m *= 2;
-  const double c = m <= 2048 ? (fp64 ? 1.15 : 0.7) : (fp64 ? 10.9 : 12.0);
+  const double c = m <= 2048 ? (fp64 ? 1.15 : 0.7) : m <= 8192 ? (fp64 ? 2.7 : 1.5) : (fp64 ? 10.9 : 12.0);
   const double blue = c * static_cast<double>(m) / n + 0.3;
   return blue < 0.8 * other;
 }

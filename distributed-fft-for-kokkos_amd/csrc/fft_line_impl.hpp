@@ -1514,6 +1514,8 @@ hipError_t dispatch_fast_f64_short(const LinePass& p, hipStream_t s);
 hipError_t dispatch_fast_f64_long(const LinePass& p, hipStream_t s);
 hipError_t dispatch_fast_f32_short(const LinePass& p, hipStream_t s);
 hipError_t dispatch_fast_f32_long(const LinePass& p, hipStream_t s);
+hipError_t dispatch_fast_f64_xl(const LinePass& p, hipStream_t s);
+hipError_t dispatch_fast_f32_xl(const LinePass& p, hipStream_t s);
 hipError_t dispatch_products_f64(const LinePass& p, hipStream_t s);
 
 }  // namespace dfft

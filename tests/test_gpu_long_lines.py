@@ -75,6 +75,28 @@ def test_long_lines_chunked_and_tuned(dev, gin, tin, tout, configure):
         assert gather_rel_l2(backs, ins) <= 1e-12
 
 
+@pytest.mark.parametrize("n", [4096, 8192])
+@pytest.mark.parametrize("fp64", [True, False])
+@pytest.mark.parametrize("real", [False, True])
+def test_fast_kernels_for_4096_and_8192(dev, n, fp64, real):
+    """Compile-time kernels of 4096 and 8192 points (one line, pairs of real
+    lines, two FP32 lines per lane) along the contiguous and a strided axis,
+    including every launch shape the options allow."""
+    for gin, tin, tout in (([4, 6, n], [2, 1, 1], [1, 2, 1]), ([n, 6, 4], [1, 2, 1], [1, 1, 2])):
+        x = seeded_input(gin, real, 29)
+        for vpt in (4, 8, 16):
+            def conf(plan, ins, outs, vpt=vpt):
+                for k in range(3):
+                    plan.set_option(f"fwd.{k}.vpt", vpt)
+                    plan.set_option(f"bwd.{k}.vpt", vpt)
+            outs, backs, ins, _ = run_gpu_plan(dev, x, gin, [0, 1, 2], tin, tout, real, fp64, "backward", 0,
+                                               configure=conf)
+            ref = reference_spectrum(x, gin, [0, 1, 2], tout, real)
+            tol = 1e-12 if fp64 else 1e-5
+            assert gather_rel_l2(outs, ref) <= tol, (gin, vpt)
+            assert gather_rel_l2(backs, ins) <= tol, (gin, vpt)
+
+
 def test_long_line_closed_form(dev):
     """A single mode exp(2 pi i m j / n) along an 8192-point axis transforms
     to n at k = m and 0 elsewhere (independent of the oracle's FFT)."""
