@@ -339,3 +339,97 @@ def test_dual_line_lanes_fp32(dev, vpt):
         _check(dev, [128, 128, 10], [0, 1, 2], [2, 1, 1], [1, 2, 1], True, False, seed=43, configure=cfg)
         _check(dev, [256, 6, 7], [0, 1, 2], [2, 1, 1], [1, 2, 1], False, False, seed=44, configure=cfg)
         _check(dev, [256, 128, 8], [0, 1, 2], [2, 2, 1], [1, 2, 2], False, False, seed=45, configure=cfg)
+
+
+_SLAB_VIEWS = {
+    # Test_SlabPlan.cpp:77-91 (View2D: one or two FFT axes), :330-341 / :1040-1052 /
+    # :2081-2093 (View3D: one, two or three axes), :2735-2752 (View4D: three axes)
+    2: ([8, 7], [[0], [1], [0, 1], [1, 0]]),
+    3: ([8, 7, 6], [[0], [1], [2], [0, 1], [0, 2], [1, 0], [1, 2], [2, 0], [2, 1]]
+        + [list(p) for p in itertools.permutations([0, 1, 2])]),
+    4: ([8, 7, 6, 6], [[0, 1, 2], [0, 2, 1], [1, 0, 2], [1, 2, 0], [2, 0, 1], [2, 1, 0],
+                       [1, 2, 3], [1, 3, 2], [2, 1, 3]]),
+}
+
+
+@pytest.mark.parametrize("view_layout", ["right", "left"])
+@pytest.mark.parametrize("fp64", [True, False])
+@pytest.mark.parametrize("ndim", [2, 3, 4])
+def test_reference_slab_matrix(dev, ndim, fp64, view_layout):
+    """The slab test matrix of Test_SlabPlan.cpp:3549-3640 (TestSlab1D / 2D / 3D x
+    View2D / 3D / 4D x R2C / C2C, typed over {float, double} x {LayoutLeft,
+    LayoutRight}): every pair of slab topologies, every axis set the reference
+    lists, on 2 and 3 virtual ranks (ragged blocks: 7 = 4 + 3, 8 = 3 + 3 + 2)."""
+    gin, axis_sets = _SLAB_VIEWS[ndim]
+    for P in (2, 3):
+        topos = []
+        for k in range(ndim):
+            t = [1] * ndim
+            t[k] = P
+            topos.append(t)
+        for tin, tout in itertools.product(topos, repeat=2):
+            for axes in axis_sets:
+                for real in (False, True):
+                    plan = _check(dev, gin, axes, tin, tout, real, fp64, seed=31 + P,
+                                  view_layout=None if view_layout == "right" else "left")
+                    assert plan.name() in ("SharedPlan", "SlabPlan")
+
+
+def _pencil_topologies(ndim, px, py):
+    """x-, y- and z-pencils of Test_PencilPlan.cpp:99-100 / :1763-1764 / :3648-3650
+    (the z-pencil is a LayoutLeft topology), with the 4-D (1, px, py, 1) pencil."""
+    if ndim == 3:
+        return [([1, px, py], "right"), ([px, 1, py], "right"), ([py, px, 1], "left")]
+    return [([1, 1, px, py], "right"), ([1, px, 1, py], "right"), ([1, py, px, 1], "left"),
+            ([1, px, py, 1], "right")]
+
+
+_PENCIL_VIEWS = {
+    # Test_PencilPlan.cpp:102-110, :587-596, :1766-1779, :2436-2447, :3653-3667
+    3: ([8, 7, 5], [[0], [1], [2], [0, 1], [0, 2], [1, 0], [1, 2], [2, 0], [2, 1]]
+        + [list(p) for p in itertools.permutations([0, 1, 2])]),
+    4: ([8, 7, 5, 6], [[0, 1], [0, 2], [1, 0], [1, 2], [2, 0], [2, 1], [1, 3], [2, 3], [0, 3]]
+        + [list(p) for p in itertools.permutations([0, 1, 2])] + [[1, 2, 3], [1, 3, 2], [2, 1, 3]]),
+}
+
+
+@pytest.mark.parametrize("view_layout", ["right", "left"])
+@pytest.mark.parametrize("fp64", [True, False])
+@pytest.mark.parametrize("ndim", [3, 4])
+def test_reference_pencil_matrix(dev, ndim, fp64, view_layout):
+    """The pencil test matrix of Test_PencilPlan.cpp:4521-4593 (TestPencil1D / 2D /
+    3D x View3D / 4D x R2C / C2C over {float, double} x {LayoutLeft, LayoutRight}),
+    every pair of the x / y / z pencils for every axis set, on 2 x 2 and 2 x 3
+    virtual ranks.  A pair whose transposes include one between equal topologies
+    of different layouts has no buffer extents in the reference
+    (Extents.hpp:35-60 merges the two topologies, ContainerAnalyses.hpp:181-226
+    refuses equal ones): the oracle's planner refuses it and so must the plan."""
+    import dfft
+    from oracle import planner as OPL
+    from helpers import out_extents
+    gin, axis_sets = _PENCIL_VIEWS[ndim]
+    vl = None if view_layout == "right" else "left"
+    refused = 0
+    for px, py in ((2, 2), (2, 3)):
+        topos = _pencil_topologies(ndim, px, py)
+        for (tin, lin), (tout, lout) in itertools.product(topos, repeat=2):
+            for axes in axis_sets:
+                for real in (False, True):
+                    try:
+                        lay = OPL.LAYOUT_LEFT if vl else OPL.LAYOUT_RIGHT
+                        gout = out_extents(gin, axes, real)
+                        ext = OPL.compute_local_extents(gin, tin, list(range(ndim)), 0,
+                                                        OPL.LAYOUT_LEFT if lin == "left" else OPL.LAYOUT_RIGHT)
+                        OPL.block_analyses(OPL.PENCIL, real, lay, ext, gin, gout, tin, tout, axes, 0,
+                                           OPL.LAYOUT_LEFT if lin == "left" else OPL.LAYOUT_RIGHT,
+                                           OPL.LAYOUT_LEFT if lout == "left" else OPL.LAYOUT_RIGHT)
+                    except OPL.PlannerError:
+                        refused += 1
+                        with pytest.raises(dfft.DFFTError, match="differ exactly two positions"):
+                            _check(dev, gin, axes, tin, tout, real, fp64, tin_layout=lin, tout_layout=lout,
+                                   view_layout=vl)
+                        continue
+                    plan = _check(dev, gin, axes, tin, tout, real, fp64, seed=41 + px + py,
+                                  tin_layout=lin, tout_layout=lout, view_layout=vl)
+                    assert plan.name() in ("SharedPlan", "SlabPlan", "PencilPlan")
+    assert refused == (8 if ndim == 4 else 0)
