@@ -301,7 +301,7 @@ def bench_mixed_batch(torch, dfft, dev, steps, warmup):
     launches = g.num_launches(dfft.Direction.forward) + g.num_launches(dfft.Direction.backward)
     pass_bytes = 2 * vol * 8  # one axis pass of every member reads and writes all of them
     passes = 6  # 3 axis passes per 3-D transform and direction
-    opts = {k: g.get_option(k) for k in ("lds_cap", "grid_mult", "nt", "dual")}
+    opts = {k: g.get_option(k) for k in ("lds_cap", "grid_mult", "nt", "dual", "lanes", "stage", "contig_nt")}
     g.destroy()
     return {"workload": "64 C2C transforms, N in {96,128,160}^3 (22/21/21) complex64, fwd+bwd, grouped, hipGraph",
             "ms": round(ms, 4), "launches": launches, "options": opts,

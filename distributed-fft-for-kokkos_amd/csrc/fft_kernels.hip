@@ -25,10 +25,13 @@ namespace dfft {
 // Grouped line pass: the same pass of many independent transforms (lengths
 // from kGroupSizes, any mix) in one launch.  Virtual blocks are numbered
 // transform by transform (first_block[i] .. first_block[i+1]); a grid of at
-// most a few blocks per CU walks them in a block-stride loop.  Every block
-// has kGroupThreads lanes; transform i uses items[i].lines_per_block lines.
+// most a few blocks per CU walks them in a block-stride loop.  Blocks have
+// kGroupThreads lanes (64 or 128 for launches of contiguous lines);
+// transform i uses items[i].lines_per_block lines.  STAGE (complex64 one-line
+// instances): contiguous, 16-byte-aligned lines move 16 bytes per lane through
+// the line's LDS buffer.
 // ---------------------------------------------------------------------------
-template <typename Real, bool NT, bool DUAL>
+template <typename Real, bool NT, bool DUAL, bool STAGE = false>
 __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* __restrict__ items,
                                                                 const int32_t* __restrict__ first_block,
                                                                 int32_t count, int32_t total_blocks) {
@@ -59,9 +62,8 @@ __global__ void __launch_bounds__(kGroupThreads) fft_group_pass(const LinePass* 
         }
       }
     } else {
-      // complex64 contiguous lines move 16 bytes per lane through LDS
-      // (complex128 elements are 16 bytes already)
-      constexpr bool kStage = std::is_same<Real, float>::value;
+      // complex128 elements are 16 bytes already: staging is for complex64
+      constexpr bool kStage = STAGE && std::is_same<Real, float>::value;
       switch (p.n) {
         case 16: fast_line_body<Real, 16, 8, NT, false, kStage>(p, blk, p.lines_per_block, lds); break;
         case 32: fast_line_body<Real, 32, 8, NT, false, kStage>(p, blk, p.lines_per_block, lds); break;
@@ -499,25 +501,26 @@ namespace {
 // lines per block: as many as the block's lanes hold, capped so the lines
 // plus the spare slot fit `cap` bytes of dynamic LDS (at most 64 KB)
 template <typename Real, int N>
-void group_shape_of(size_t cap, int* B, size_t* lds) {
+void group_shape_of(size_t cap, int lanes, int* B, size_t* lds) {
   using Cfg = FastCfg<Real, N, 8>;
+  static_assert(Cfg::T <= 64, "a grouped line fits the smallest block");
   const size_t line_bytes = static_cast<size_t>(Cfg::LDSN) * sizeof(cplx_t<Real>);
   if (cap > kGroupLds) cap = kGroupLds;
-  int b = kGroupThreads / Cfg::T;
+  int b = lanes / Cfg::T;
   while (b > 1 && static_cast<size_t>(b + 1) * line_bytes > cap) --b;
   *B = b;
   *lds = static_cast<size_t>(b + 1) * line_bytes;
 }
 template <typename Real>
-bool group_shape(int n, size_t cap, int* B, size_t* lds) {
+bool group_shape(int n, size_t cap, int lanes, int* B, size_t* lds) {
   switch (n) {
-    case 16: group_shape_of<Real, 16>(cap, B, lds); return true;
-    case 32: group_shape_of<Real, 32>(cap, B, lds); return true;
-    case 64: group_shape_of<Real, 64>(cap, B, lds); return true;
-    case 96: group_shape_of<Real, 96>(cap, B, lds); return true;
-    case 128: group_shape_of<Real, 128>(cap, B, lds); return true;
-    case 160: group_shape_of<Real, 160>(cap, B, lds); return true;
-    case 256: group_shape_of<Real, 256>(cap, B, lds); return true;
+    case 16: group_shape_of<Real, 16>(cap, lanes, B, lds); return true;
+    case 32: group_shape_of<Real, 32>(cap, lanes, B, lds); return true;
+    case 64: group_shape_of<Real, 64>(cap, lanes, B, lds); return true;
+    case 96: group_shape_of<Real, 96>(cap, lanes, B, lds); return true;
+    case 128: group_shape_of<Real, 128>(cap, lanes, B, lds); return true;
+    case 160: group_shape_of<Real, 160>(cap, lanes, B, lds); return true;
+    case 256: group_shape_of<Real, 256>(cap, lanes, B, lds); return true;
     default: return false;
   }
 }
@@ -550,28 +553,33 @@ bool group_dual_shape(int n, size_t lds_cap, int* lines_per_block, size_t* lds_b
 
 bool dual_lines_apply(const LinePass& p, int lines_per_block) { return dual_lines_apply_impl(p, lines_per_block); }
 
-bool group_line_shape(int n, bool fp64, size_t lds_cap, int* lines_per_block, size_t* lds_bytes) {
-  return fp64 ? group_shape<double>(n, lds_cap, lines_per_block, lds_bytes)
-              : group_shape<float>(n, lds_cap, lines_per_block, lds_bytes);
+bool group_line_shape(int n, bool fp64, size_t lds_cap, int lanes, int* lines_per_block, size_t* lds_bytes) {
+  if (lanes != 64 && lanes != 128 && lanes != kGroupThreads) return false;
+  return fp64 ? group_shape<double>(n, lds_cap, lanes, lines_per_block, lds_bytes)
+              : group_shape<float>(n, lds_cap, lanes, lines_per_block, lds_bytes);
 }
 
-int group_grid(size_t lds_bytes, bool fp64, bool dual) {
+int group_grid(size_t lds_bytes, bool fp64, bool dual, int lanes) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   const void* f = fp64   ? reinterpret_cast<const void*>(&fft_group_pass<double, false, false>)
                   : dual ? reinterpret_cast<const void*>(&fft_group_pass<float, false, true>)
-                         : reinterpret_cast<const void*>(&fft_group_pass<float, false, false>);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kGroupThreads, lds_bytes) != hipSuccess) return 0;
+                         : reinterpret_cast<const void*>(&fft_group_pass<float, false, false, true>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, dual ? kGroupThreads : lanes, lds_bytes) != hipSuccess)
+    return 0;
   return cus * (per_cu > 0 ? per_cu : 1);
 }
 
 hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
-                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, bool dual, hipStream_t s) {
+                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, bool dual, int lanes,
+                             bool stage, hipStream_t s) {
   if (total_blocks <= 0 || count <= 0) return hipSuccess;
   if (grid <= 0 || lds_bytes > kGroupLds || (dual && fp64)) return hipErrorInvalidConfiguration;
+  if (lanes != 64 && lanes != 128 && lanes != kGroupThreads) return hipErrorInvalidConfiguration;
+  if (dual && lanes != kGroupThreads) return hipErrorInvalidConfiguration;
   const dim3 g(static_cast<unsigned>(grid < total_blocks ? grid : total_blocks));
-  const dim3 b(kGroupThreads);
+  const dim3 b(static_cast<unsigned>(lanes));
   if (fp64) {
     if (nontemporal)
       hipLaunchKernelGGL((fft_group_pass<double, true, false>), g, b, lds_bytes, s, items, first_block, count,
@@ -586,6 +594,13 @@ hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, 
     else
       hipLaunchKernelGGL((fft_group_pass<float, false, true>), g, b, lds_bytes, s, items, first_block, count,
                          total_blocks);
+  } else if (stage) {
+    if (nontemporal)
+      hipLaunchKernelGGL((fft_group_pass<float, true, false, true>), g, b, lds_bytes, s, items, first_block, count,
+                         total_blocks);
+    else
+      hipLaunchKernelGGL((fft_group_pass<float, false, false, true>), g, b, lds_bytes, s, items, first_block,
+                         count, total_blocks);
   } else {
     if (nontemporal)
       hipLaunchKernelGGL((fft_group_pass<float, true, false>), g, b, lds_bytes, s, items, first_block, count,

@@ -98,13 +98,16 @@ struct RemapArgs {
 };
 
 // Grouped line pass (one launch for the same pass of many transforms, line
-// lengths 16, 32, 64, 96, 128, 160, 256 in any mix).
+// lengths 16, 32, 64, 96, 128, 160, 256 in any mix).  kGroupThreads lanes per
+// block; launches of contiguous lines (lanes along the line) may use fewer
+// (64 or 128: smaller blocks, cheaper barriers, more blocks per CU).
 constexpr int kGroupThreads = 256;
 constexpr size_t kGenericLdsMax = 160 * 1024;  // LDS of one gfx950 workgroup
 constexpr size_t kGroupLds = 64 * 1024;  // dynamic LDS of a grouped launch, at most
 // lines per block and dynamic LDS bytes of the grouped kernel for length n
-// with at most lds_cap bytes of LDS per block; false if n has no grouped kernel
-bool group_line_shape(int n, bool fp64, size_t lds_cap, int* lines_per_block, size_t* lds_bytes);
+// in blocks of `lanes` lanes (64, 128 or kGroupThreads) with at most lds_cap
+// bytes of LDS per block; false if n has no grouped kernel
+bool group_line_shape(int n, bool fp64, size_t lds_cap, int lanes, int* lines_per_block, size_t* lds_bytes);
 // the same for FP32 lines taken two per lane (false if n has no such kernel)
 bool group_dual_shape(int n, size_t lds_cap, int* lines_per_block, size_t* lds_bytes);
 // true if two adjacent lines per lane apply to pass p with that many lines per
@@ -112,10 +115,13 @@ bool group_dual_shape(int n, size_t lds_cap, int* lines_per_block, size_t* lds_b
 // elements on both sides, 16-byte aligned rank buffers; grouped passes)
 bool dual_lines_apply(const LinePass& p, int lines_per_block);
 // grid of the persistent grouped launch: resident blocks of the whole device
-int group_grid(size_t lds_bytes, bool fp64, bool dual);
-// dual: the launch holds only members with dual_lines set (FP32)
+int group_grid(size_t lds_bytes, bool fp64, bool dual, int lanes);
+// dual: the launch holds only members with dual_lines set (FP32, kGroupThreads
+// lanes); lanes: lanes per block of a one-line launch; stage (FP32 one-line):
+// contiguous aligned lines move 16 bytes per lane through LDS
 hipError_t launch_group_pass(const LinePass* items, const int32_t* first_block, int count, int total_blocks,
-                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, bool dual, hipStream_t s);
+                             int grid, size_t lds_bytes, bool fp64, bool nontemporal, bool dual, int lanes,
+                             bool stage, hipStream_t s);
 bool has_fast_line_kernel(int n);
 // true if a line of length n runs as one pass: the fast set, or the generic
 // kernel (one line of n complex values, twice, in at most 160 KB of LDS)

@@ -193,7 +193,7 @@ std::vector<LinePass> Group::build_pass(const std::vector<void*>& in, const std:
     int B = 0;
     std::size_t lds = 0;
     p.values_per_thread = 8;
-    p.lines_per_block = group_line_shape(p.n, fp64_, lds_cap_, &B, &lds) ? B : 0;
+    p.lines_per_block = group_line_shape(p.n, fp64_, lds_cap_, kGroupThreads, &B, &lds) ? B : 0;
     if (!fp64_ && dual_ && kind == kC2C && p.lines_fast && group_dual_shape(p.n, lds_cap_, &B, &lds)) {
       LinePass q = p;
       q.dual_lines = 1;
@@ -217,6 +217,7 @@ void Group::build_tables(const std::vector<void*>& in, const std::vector<void*>&
   t.grid.assign(ns, 0);
   t.nt.assign(ns, 0);
   t.lds.assign(ns, 0);
+  t.lanes.assign(ns, kGroupThreads);
   t.single.assign(np, {});
   for (std::size_t k = 0; k < np; ++k) {
     std::vector<LinePass> all = build_pass(in, out, dir == 0, static_cast<int>(k));
@@ -231,18 +232,17 @@ void Group::build_tables(const std::vector<void*>& in, const std::vector<void*>&
     std::vector<LinePass> grouped[2];
     std::vector<int32_t> first[2] = {{0}, {0}};
     std::size_t lds[2] = {0, 0};
+    bool contiguous = true;  // every one-line member walks its lines with lanes along the line
     for (const LinePass& p : all) {
       int B = 0;
       std::size_t l = 0;
       if (p.nlines == 0) continue;
       const int v = p.dual_lines ? 1 : 0;
-      const bool shaped = v ? group_dual_shape(p.n, lds_cap_, &B, &l) : group_line_shape(p.n, fp64_, lds_cap_, &B, &l);
+      const bool shaped = v ? group_dual_shape(p.n, lds_cap_, &B, &l)
+                            : group_line_shape(p.n, fp64_, lds_cap_, kGroupThreads, &B, &l);
       if (shaped) {
         grouped[v].push_back(p);
-        const int64_t nb = first[v].back() + (static_cast<int64_t>(p.nlines) + B - 1) / B;
-        if (nb >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many blocks in one grouped pass");
-        first[v].push_back(static_cast<int32_t>(nb));
-        lds[v] = std::max(lds[v], l);
+        if (v == 0 && p.lines_fast) contiguous = false;
       } else {
         LinePass q = p;
         q.lines_per_block = 0;  // launcher heuristic
@@ -252,13 +252,28 @@ void Group::build_tables(const std::vector<void*>& in, const std::vector<void*>&
       }
     }
     for (int v = 0; v < 2; ++v) {
+      // a one-line launch of contiguous lines only takes lanes_ lanes per block;
+      // strided lines keep full blocks (more adjacent lines per access)
+      const int lanes = v == 0 && contiguous ? lanes_ : kGroupThreads;
+      for (LinePass& p : grouped[v]) {
+        int B = 0;
+        std::size_t l = 0;
+        if (v) group_dual_shape(p.n, lds_cap_, &B, &l);
+        else group_line_shape(p.n, fp64_, lds_cap_, lanes, &B, &l);
+        p.lines_per_block = B;
+        const int64_t nb = first[v].back() + (static_cast<int64_t>(p.nlines) + B - 1) / B;
+        if (nb >= (int64_t(1) << 31)) throw Error(DFFT_ERROR_UNSUPPORTED, "too many blocks in one grouped pass");
+        first[v].push_back(static_cast<int32_t>(nb));
+        lds[v] = std::max(lds[v], l);
+      }
       const std::size_t slot = 2 * k + static_cast<std::size_t>(v);
-      t.nt[slot] = nt;
+      t.nt[slot] = v == 0 && contiguous && contig_nt_ >= 0 ? contig_nt_ : nt;
+      t.lanes[slot] = lanes;
       t.count[slot] = static_cast<int>(grouped[v].size());
       t.total[slot] = first[v].back();
       t.lds[slot] = lds[v];
       if (grouped[v].empty()) continue;
-      const int resident = group_grid(lds[v], fp64_, v == 1);
+      const int resident = group_grid(lds[v], fp64_, v == 1, lanes);
       if (resident <= 0) throw Error(DFFT_ERROR_HIP, "occupancy query of the grouped kernel failed");
       t.grid[slot] = grid_mult_ > 0 ? resident * grid_mult_ : first[v].back();
       t.items[slot] = std::move(grouped[v]);
@@ -300,7 +315,7 @@ void Group::launch_tables(const Tables& t, LinePass* const* items, int32_t* cons
       const int slot = 2 * k + v;
       if (t.count[slot] > 0)
         check(launch_group_pass(items[slot], first[slot], t.count[slot], t.total[slot], t.grid[slot], t.lds[slot],
-                                fp64_, t.nt[slot] != 0, v == 1, s),
+                                fp64_, t.nt[slot] != 0, v == 1, t.lanes[slot], stage_ != 0, s),
               "grouped line pass");
     }
     for (const LinePass& p : t.single[k]) check(launch_line_pass(p, fp64_, s), "line pass");
@@ -331,7 +346,7 @@ int Group::num_launches(int direction) const {
       const std::vector<int>& order = direction == DFFT_FORWARD ? m.fwd_axes : m.bwd_axes;
       int B = 0;
       std::size_t l = 0;
-      if (group_line_shape(static_cast<int>(m.gin[order[k]]), fp64_, lds_cap_, &B, &l)) grouped = true;
+      if (group_line_shape(static_cast<int>(m.gin[order[k]]), fp64_, lds_cap_, kGroupThreads, &B, &l)) grouped = true;
       else ++n;
     }
     if (grouped) ++n;
@@ -368,6 +383,16 @@ void Group::set_option(const std::string& key, long value) {
   } else if (key == "dual") {
     if (value < 0 || value > 1) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "dual must be 0 or 1");
     dual_ = static_cast<int>(value);
+  } else if (key == "contig_nt") {
+    if (value < -1 || value > 1) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "contig_nt must be -1, 0 or 1");
+    contig_nt_ = static_cast<int>(value);
+  } else if (key == "stage") {
+    if (value < 0 || value > 1) throw Error(DFFT_ERROR_INVALID_ARGUMENT, "stage must be 0 or 1");
+    stage_ = static_cast<int>(value);
+  } else if (key == "lanes") {
+    if (value != 64 && value != 128 && value != kGroupThreads)
+      throw Error(DFFT_ERROR_INVALID_ARGUMENT, "lanes must be 64, 128 or 256");
+    lanes_ = static_cast<int>(value);
   } else {
     throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown group option: " + key);
   }
@@ -379,6 +404,9 @@ long Group::get_option(const std::string& key) const {
   if (key == "grid_mult") return grid_mult_;
   if (key == "nt") return nt_;
   if (key == "dual") return dual_;
+  if (key == "lanes") return lanes_;
+  if (key == "stage") return stage_;
+  if (key == "contig_nt") return contig_nt_;
   throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown group option: " + key);
 }
 
@@ -388,8 +416,32 @@ void Group::autotune(void* const* in, void* const* out, hipStream_t s) {
   check(hipEventCreate(&e1), "hipEventCreate");
   float best = -1.0f;
   std::size_t best_cap = lds_cap_;
-  int best_mult = grid_mult_, best_nt = nt_, best_dual = dual_;
+  int best_mult = grid_mult_, best_nt = nt_, best_dual = dual_, best_lanes = lanes_, best_stage = stage_;
+  int best_cnt = contig_nt_;
+  // one configuration: warm-up round trip (builds the tables), then the best
+  // of two trials of three round trips (single trials are noisy)
+  auto time_config = [&]() {
+    execute(in, out, DFFT_FORWARD, false, s);
+    execute(out, in, DFFT_BACKWARD, false, s);
+    float ms = -1.0f;
+    for (int trial = 0; trial < 2; ++trial) {
+      check(hipEventRecord(e0, s), "hipEventRecord");
+      for (int rep = 0; rep < 3; ++rep) {
+        execute(in, out, DFFT_FORWARD, false, s);
+        execute(out, in, DFFT_BACKWARD, false, s);
+      }
+      check(hipEventRecord(e1, s), "hipEventRecord");
+      check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float t = 0.0f;
+      check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+      if (ms < 0.0f || t < ms) ms = t;
+    }
+    return ms;
+  };
   try {
+    set_option("lanes", kGroupThreads);
+    set_option("stage", 1);
+    set_option("contig_nt", -1);
     for (std::size_t cap : {std::size_t(65536), std::size_t(49152), std::size_t(32768), std::size_t(24576)}) {
       for (int multnt = 0; multnt < (fp64_ ? 8 : 16); ++multnt) {
         const int mult = (multnt & 3) == 3 ? 0 : (1 << (multnt & 3));
@@ -399,28 +451,42 @@ void Group::autotune(void* const* in, void* const* out, hipStream_t s) {
         set_option("grid_mult", mult);
         set_option("nt", nt);
         set_option("dual", dual);
-        execute(in, out, DFFT_FORWARD, false, s);  // builds the tables (warm-up)
-        execute(out, in, DFFT_BACKWARD, false, s);
-        // best of two trials of three round trips: single trials are noisy
-        float ms = -1.0f;
-        for (int trial = 0; trial < 2; ++trial) {
-          check(hipEventRecord(e0, s), "hipEventRecord");
-          for (int rep = 0; rep < 3; ++rep) {
-            execute(in, out, DFFT_FORWARD, false, s);
-            execute(out, in, DFFT_BACKWARD, false, s);
-          }
-          check(hipEventRecord(e1, s), "hipEventRecord");
-          check(hipEventSynchronize(e1), "hipEventSynchronize");
-          float t = 0.0f;
-          check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
-          if (ms < 0.0f || t < ms) ms = t;
-        }
+        const float ms = time_config();
         if (best < 0.0f || ms < best) {
           best = ms;
           best_cap = cap;
           best_mult = mult;
           best_nt = nt;
           best_dual = dual;
+        }
+      }
+    }
+    // then the launches of contiguous lines: block size, FP32 staging and
+    // their own access kind, with every grid size (the LDS cap, the dual
+    // choice and the other launches' access kind found above)
+    set_option("lds_cap", static_cast<long>(best_cap));
+    set_option("dual", best_dual);
+    set_option("nt", best_nt);
+    best_lanes = kGroupThreads;
+    best_stage = 1;
+    best_cnt = -1;
+    for (int stage = fp64_ ? 1 : 0; stage < 2; ++stage) {
+      for (int lanes : {256, 128, 64}) {
+        for (int multnt = 0; multnt < 8; ++multnt) {
+          const int mult = (multnt & 3) == 3 ? 0 : (1 << (multnt & 3));
+          const int cnt = multnt >> 2;
+          set_option("stage", stage);
+          set_option("lanes", lanes);
+          set_option("grid_mult", mult);
+          set_option("contig_nt", cnt);
+          const float ms = time_config();
+          if (ms < best) {
+            best = ms;
+            best_stage = stage;
+            best_lanes = lanes;
+            best_mult = mult;
+            best_cnt = cnt;
+          }
         }
       }
     }
@@ -435,6 +501,9 @@ void Group::autotune(void* const* in, void* const* out, hipStream_t s) {
   set_option("grid_mult", best_mult);
   set_option("nt", best_nt);
   set_option("dual", best_dual);
+  set_option("lanes", best_lanes);
+  set_option("stage", best_stage);
+  set_option("contig_nt", best_cnt);
 }
 
 int Group::check_call(const std::vector<void*>& vin, const std::vector<void*>& vout, int direction) const {

@@ -61,7 +61,7 @@ class Group {
   struct Tables {
     std::vector<std::vector<LinePass>> items;
     std::vector<std::vector<int32_t>> first;  // virtual-block prefix per slot
-    std::vector<int> count, total, grid, nt;
+    std::vector<int> count, total, grid, nt, lanes;
     std::vector<size_t> lds;
     std::vector<std::vector<LinePass>> single;  // per pass: individually launched passes
   };
@@ -84,6 +84,9 @@ class Group {
   int grid_mult_ = 1;
   int nt_ = -1;
   int dual_ = 1;  // FP32: two adjacent lines per lane where they apply
+  int lanes_ = kGroupThreads;  // lanes per block of launches that hold only contiguous lines
+  int stage_ = 1;  // FP32: contiguous aligned lines 16 bytes per lane through LDS
+  int contig_nt_ = -1;  // nontemporal accesses of launches of contiguous lines (-1: as nt_)
   std::map<int, void*> tw_;
   // per direction: the pointer set the device tables were built for, the
   // tables of every pass (LinePass items + virtual-block prefix), members

@@ -220,9 +220,13 @@ int dfft_group_execute_noalloc(dfft_group* group, void* const* in, void* const* 
 int dfft_group_num_launches(const dfft_group* group, int direction, int* nlaunches);
 /* "lds_cap" (LDS bytes per block of the grouped kernel, 1024..65536),
  * "grid_mult" (multiples of the resident-block grid; 0 = one block per
- * virtual block), "nt" (nontemporal accesses: -1 auto, 0, 1) and "dual"
+ * virtual block), "nt" (nontemporal accesses: -1 auto, 0, 1), "dual"
  * (FP32: two adjacent lines per lane on strided passes where they apply; 1
- * default, 0 off); the autotuner times combinations on the caller's arrays
+ * default, 0 off), "lanes" (64, 128 or 256 lanes per block for the
+ * launches whose lines are all contiguous; 256 default) and "stage" (FP32:
+ * contiguous 16-byte-aligned lines move 16 bytes per lane through LDS; 1
+ * default, 0 off) and "contig_nt" (nontemporal accesses of those launches:
+ * -1 as "nt", 0, 1); the autotuner times combinations on the caller's arrays
  * (contents are overwritten) and keeps the fastest. */
 int dfft_group_set_option(dfft_group* group, const char* key, long value);
 int dfft_group_get_option(const dfft_group* group, const char* key, long* value);

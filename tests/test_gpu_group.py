@@ -192,3 +192,50 @@ def test_group_dual_lines(dev):
         launches[dual] = g.num_launches(dfft.Direction.forward)
         g.destroy()
     assert launches[1] == launches[0] + 2
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32-staged", "fp32"])
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("lanes", [64, 128, 256])
+def test_group_contiguous_lanes(dev, lanes, real, prec):
+    """Launches of contiguous lines in blocks of 64, 128 or 256 lanes (every
+    grouped length: 2 to 32 lanes per line, so 2 to 32 lines per block), with
+    persistent and one-block-per-virtual-block grids, FP32 with and without
+    the LDS staging of contiguous lines, cached and nontemporal accesses of
+    those launches, against the oracle.  The
+    first pass of these members walks contiguous lines only; with the
+    (16, 64, 5) member over axes [0, 1] added, its strided axis-1 lines join
+    that launch, which then keeps 256-lane blocks."""
+    import torch
+    import dfft
+    shapes = [[16, 16, 16], [8, 32, 32], [4, 6, 64], [12, 96, 8], [4, 8, 128], [160, 8, 4], [2, 4, 256],
+              [6, 5, 160], [16, 64], [20, 96]]
+    axes_list = [[0, 1, 2]] * 8 + [[0, 1], [1]]
+    fp64 = prec == "fp64"
+    tol = 1e-12 if fp64 else 2e-6
+    for mult, extra in ((0, False), (1, False), (0, True)):
+        sh = shapes + ([[16, 64, 5]] if extra else [])
+        ax = axes_list + ([[0, 1]] if extra else [])
+        xs, ins, outs = _members(dev, sh, real, fp64, ax, 50 + lanes)
+        g = dfft.Group(ins, outs, ax)
+        g.set_option("lanes", lanes)
+        g.set_option("grid_mult", mult)
+        g.set_option("stage", 0 if prec == "fp32" else 1)
+        g.set_option("contig_nt", mult)
+        assert g.get_option("lanes") == lanes and g.get_option("contig_nt") == mult
+        g.execute(ins, outs, dfft.Direction.forward)
+        backs = [torch.full(list(t.shape), float("nan"), dtype=t.dtype, device=dev) for t in ins]
+        g.execute(outs, backs, dfft.Direction.backward)
+        torch.cuda.synchronize()
+        for x, o, b, axes in zip(xs, outs, backs, ax):
+            what = f"lanes={lanes} mult={mult} extra={extra} {x.shape}"
+            assert rel_l2(o.cpu().numpy(), F.fftn(x, axes, real)) <= tol, what
+            assert rel_l2(b.cpu().numpy(), x) <= tol, what
+        if extra:
+            with pytest.raises(dfft.DFFTError):
+                g.set_option("lanes", 96)
+            with pytest.raises(dfft.DFFTError):
+                g.set_option("stage", 2)
+            with pytest.raises(dfft.DFFTError):
+                g.set_option("contig_nt", 2)
+        g.destroy()
