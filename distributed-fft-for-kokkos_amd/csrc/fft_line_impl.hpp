@@ -147,14 +147,20 @@ template <int r, typename C> __device__ __forceinline__ void bfly(C* a) {
 // ---------------------------------------------------------------------------
 // Global element access: nontemporal (streaming) or cached, chosen at
 // compile time (a runtime choice between the two forms of one access is
-// folded into the cached form by the compiler).
+// folded into the cached form by the compiler).  The pointer is taken as a
+// global-memory one: line addresses read from a table in device memory (the
+// grouped kernel) are otherwise generic and compile to flat instructions,
+// which also count against the LDS wait counter.
+template <typename T> using global_ptr = __attribute__((address_space(1))) T*;
 template <bool NT, typename T> __device__ __forceinline__ T gload(const T* p) {
-  if constexpr (NT) return __builtin_nontemporal_load(p);
-  else return *p;
+  const global_ptr<const T> g = (global_ptr<const T>)p;
+  if constexpr (NT) return __builtin_nontemporal_load(g);
+  else return *g;
 }
 template <bool NT, typename T> __device__ __forceinline__ void gstore(T* p, T v) {
-  if constexpr (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
+  const global_ptr<T> g = (global_ptr<T>)p;
+  if constexpr (NT) __builtin_nontemporal_store(v, g);
+  else *g = v;
 }
 template <bool NT, typename Real> __device__ __forceinline__ cplx_t<Real> gload_c(const cplx_t<Real>* p) {
   typedef Real V2 __attribute__((ext_vector_type(2)));
@@ -398,6 +404,15 @@ template <> __device__ __forceinline__ float2 to_prec<float2>(double2 w) {
   return float2{static_cast<float>(w.x), static_cast<float>(w.y)};
 }
 
+// One twiddle-table entry, read as a native vector from global memory (a
+// double2 is a class: copying it through a global pointer would pass a
+// generic reference and compile to a flat load).
+__device__ __forceinline__ double2 twiddle_load(const double2* p) {
+  typedef double D2 __attribute__((ext_vector_type(2)));
+  const D2 w = gload<false>(reinterpret_cast<const D2*>(p));
+  return double2{w.x, w.y};
+}
+
 template <typename Real, int N, int RM, int S>
 __device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const double2* tw) {
   using Cfg = FastCfg<Real, N, RM>;
@@ -413,7 +428,7 @@ __device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const doub
       const int k = beta % L;
       constexpr int ts = N / (L * r);
       double2 pw[8];
-      twiddle_powers<r>(tw[k * ts], pw);
+      twiddle_powers<r>(twiddle_load(tw + k * ts), pw);
 #pragma unroll
       for (int j = 1; j < r; ++j) v[q * r + j] = cmul(v[q * r + j], to_prec<C>(pw[j]));
     }
@@ -557,7 +572,7 @@ __device__ __forceinline__ void stage_compute2(cplx_t<Real>* va, cplx_t<Real>* v
       const int k = beta % L;
       constexpr int ts = N / (L * r);
       double2 pw[8];
-      twiddle_powers<r>(tw[k * ts], pw);
+      twiddle_powers<r>(twiddle_load(tw + k * ts), pw);
 #pragma unroll
       for (int j = 1; j < r; ++j) {
         const C w = to_prec<C>(pw[j]);
