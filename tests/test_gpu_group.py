@@ -81,7 +81,10 @@ def test_group_unaligned_complex64_members(dev):
     g.destroy()
 
 
-def test_group_graph_replay_matches_eager(dev):
+@pytest.mark.parametrize("lanes,stage", [(256, 1), (64, 0), (64, 1)])
+def test_group_graph_replay_matches_eager(dev, lanes, stage):
+    """Graph replays equal the eager launches bit for bit, every replay, also
+    with the contiguous launches in 64-lane blocks, staged or not."""
     import torch
     import dfft
     shapes = [[32, 32, 32], [16, 96, 16], [160, 16, 8], [24, 16, 16]]
@@ -89,6 +92,8 @@ def test_group_graph_replay_matches_eager(dev):
     xs, ins, outs = _members(dev, shapes, False, False, axes_list, 20)
     eager = [torch.empty_like(t) for t in outs]
     g = dfft.Group(ins, outs, axes_list, norm="ortho")
+    g.set_option("lanes", lanes)
+    g.set_option("stage", stage)
     g.execute(ins, eager, dfft.Direction.forward)
     for rep in range(4):
         for t in outs:

@@ -108,3 +108,34 @@ def test_no_private_memory_in_power_of_two_line_kernels(kernels):
         elif priv > 512:
             big.append((name, priv))
     assert not big, f"kernels with private memory: {big}"
+
+
+OBJDUMP = Path("/opt/rocm/llvm/bin/llvm-objdump")
+
+
+def test_no_flat_memory_instructions():
+    """Every global access of the kernels compiles to global_* instructions.
+    A flat access (an address the compiler cannot place in global memory, e.g.
+    a pointer read from a table in device memory) also counts against the LDS
+    wait counter, so each LDS wait of the radix stages would wait for the line
+    loads in flight too (the grouped kernel ran 12 % slower that way)."""
+    import subprocess
+    import tempfile
+    if not LIB.exists():
+        pytest.skip("libdfft.so is not built")
+    if not OBJDUMP.exists():
+        pytest.skip("llvm-objdump is not installed")
+    flat = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for i, co in enumerate(_code_objects(LIB.read_bytes())):
+            path = Path(tmp) / f"co{i}.o"
+            path.write_bytes(co)
+            out = subprocess.run([str(OBJDUMP), "-d", "--no-show-raw-insn", str(path)], check=True,
+                                 capture_output=True, text=True).stdout
+            func = None
+            for line in out.splitlines():
+                if line.endswith(">:"):
+                    func = line[line.find("<") + 1: -2]
+                elif "flat_load" in line or "flat_store" in line or "flat_atomic" in line:
+                    flat[func] = flat.get(func, 0) + 1
+    assert not flat, f"flat memory instructions: {sorted(flat.items())[:10]}"
