@@ -846,17 +846,28 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
   if (vin) {
     // loaded above, through LDS
   } else if (p.kind == kC2C) {
-    // C2C lines are walked with pointer increments (no 64-bit product per element)
+    // C2C lines are walked with pointer increments (no 64-bit product per
+    // element).  One branch around all the loads: a select per load puts
+    // each load under its own exec mask, and the zeroing moves between them
+    // made the compiler wait for the loads in flight halfway through.
+    if (valid) {
 #pragma unroll
-    for (int q = 0; q < NB0; ++q) {
-      const C* src = reinterpret_cast<const C*>(io.ib) + static_cast<int64_t>(t * NB0 + q) * io.is;
-      const int64_t step = static_cast<int64_t>(N / r0) * io.is;
+      for (int q = 0; q < NB0; ++q) {
+        const C* src = reinterpret_cast<const C*>(io.ib) + static_cast<int64_t>(t * NB0 + q) * io.is;
+        const int64_t step = static_cast<int64_t>(N / r0) * io.is;
 #pragma unroll
-      for (int j = 0; j < r0; ++j) {
-        C x = valid ? gload_c<NT, Real>(src) : C{Real(0), Real(0)};
-        v[q * r0 + j] = p.inverse ? cconj(x) : x;
-        src += step;
+        for (int j = 0; j < r0; ++j) {
+          v[q * r0 + j] = gload_c<NT, Real>(src);
+          src += step;
+        }
       }
+      if (p.inverse) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = cconj(v[j]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = C{Real(0), Real(0)};
     }
   } else {
 #pragma unroll
@@ -1027,19 +1038,38 @@ __device__ __forceinline__ void fast_pair_body(const LinePass& p, int32_t blk, i
         // lane holding position m; that lane also forms the input at the
         // mirrored position n - m (from conj X[m]) and hands it over through
         // LDS below
+        // beta < S, so positions beta + j*S with j < r0/2 are below n/2, the
+        // one with j = r0/2 reaches n/2 only for beta = 0 and the others lie
+        // above: the loads of the first r0/2 positions are unconditional and
+        // sit under one branch per line (a branch per load would wait for
+        // each pair of loads before the next)
+        constexpr int JH = r0 / 2;
         const C* la = reinterpret_cast<const C*>(io.ib[0]) + static_cast<int64_t>(beta) * io.is[0];
         const C* lb = reinterpret_cast<const C*>(io.ib[1]) + static_cast<int64_t>(beta) * io.is[1];
+        const bool mid = 2 * (beta + JH * S) <= N;
+        C A[JH + 1], Bv[JH + 1];
 #pragma unroll
-        for (int j = 0; j < r0; ++j) {
+        for (int j = 0; j <= JH; ++j) {
+          A[j] = C{Real(0), Real(0)};
+          Bv[j] = C{Real(0), Real(0)};
+        }
+        if (valid) {
+#pragma unroll
+          for (int j = 0; j < JH; ++j) A[j] = gload_c<NT, Real>(la + j * sa);
+          if (mid) A[JH] = gload_c<NT, Real>(la + JH * sa);
+        }
+        if (vb) {
+#pragma unroll
+          for (int j = 0; j < JH; ++j) Bv[j] = gload_c<NT, Real>(lb + j * sb);
+          if (mid) Bv[JH] = gload_c<NT, Real>(lb + JH * sb);
+        }
+#pragma unroll
+        for (int j = 0; j <= JH; ++j) {
           const int pos = beta + j * S;
-          if (2 * pos <= N) {
-            const C A = valid ? gload_c<NT, Real>(la) : C{Real(0), Real(0)};
-            const C Bv = vb ? gload_c<NT, Real>(lb) : C{Real(0), Real(0)};
-            v[q * r0 + j] = io.c2r_value(pos, true, A, Bv);
-            mirror[q * r0 + j] = io.c2r_value(N - pos, false, A, Bv);
+          if (j < JH || mid) {
+            v[q * r0 + j] = io.c2r_value(pos, true, A[j], Bv[j]);
+            mirror[q * r0 + j] = io.c2r_value(N - pos, false, A[j], Bv[j]);
           }
-          la += sa;
-          lb += sb;
         }
       }
     }
