@@ -191,12 +191,12 @@ template <typename Real, bool NT = false> struct LineIO {
     } else if (kind == kR2C) {
       v = C{gload<NT>(reinterpret_cast<const Real*>(ib) + pos * is), Real(0)};
     } else {
+      // the address is selected, not the load: loads under a branch each
+      // would be waited for one by one
       const int32_t nh = n / 2 + 1;
-      if (pos < nh) {
-        v = gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + pos * is);
-      } else {
-        v = cconj(gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + (n - pos) * is));
-      }
+      const bool upper = pos >= nh;
+      v = gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + (upper ? n - pos : pos) * is);
+      if (upper) v = cconj(v);
       if (pos == 0 || 2 * pos == n) {
         v.y = Real(0);
       } else {
@@ -869,13 +869,17 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
 #pragma unroll
       for (int j = 0; j < R; ++j) v[j] = C{Real(0), Real(0)};
     }
-  } else {
+  } else if (valid) {
+    // R2C / C2R lines (one branch around the loads, as for C2C)
 #pragma unroll
     for (int q = 0; q < NB0; ++q) {
       const int beta = t * NB0 + q;
 #pragma unroll
-      for (int j = 0; j < r0; ++j) v[q * r0 + j] = valid ? io.load(beta + j * (N / r0)) : C{Real(0), Real(0)};
+      for (int j = 0; j < r0; ++j) v[q * r0 + j] = io.load(beta + j * (N / r0));
     }
+  } else {
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = C{Real(0), Real(0)};
   }
   StageLoop<Real, N, RM, SPLIT, 0>::run(v, t, tw, buf);
   bool vout = false;

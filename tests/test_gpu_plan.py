@@ -433,3 +433,23 @@ def test_reference_pencil_matrix(dev, ndim, fp64, view_layout):
                                   tin_layout=lin, tout_layout=lout, view_layout=vl)
                     assert plan.name() in ("SharedPlan", "SlabPlan", "PencilPlan")
     assert refused == (8 if ndim == 4 else 0)
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+@pytest.mark.parametrize("lf", [0, 1])
+def test_unpaired_real_lines(dev, fp64, lf):
+    """R2C / C2R passes with one real line per complex FFT (`pair` off: the
+    one-line kernel loads reals for R2C and the Hermitian extension of the
+    half spectrum for C2R), lanes along or across the lines, every fast
+    length class, against the oracle."""
+    def cfg(plan, ins, outs):
+        for d in ("fwd", "bwd"):
+            for i in range(plan.num_passes(0 if d == "fwd" else 1)):
+                plan.set_option(f"{d}.{i}.pair", 0)
+                plan.set_option(f"{d}.{i}.lf", lf)
+    for gin, axes, tin, tout in (([8, 6, 512], [0, 1, 2], [2, 1, 1], [1, 2, 1]),
+                                 ([6, 96, 10], [2, 0, 1], [1, 1, 2], [2, 1, 1]),
+                                 ([160, 4, 6], [1, 2, 0], [2, 1, 1], [1, 2, 1]),
+                                 ([16, 1024, 3], [0, 2, 1], [1, 2, 1], [2, 1, 1]),
+                                 ([12, 16, 32], [0, 1, 2], [2, 2, 1], [1, 2, 2])):
+        _check(dev, gin, axes, tin, tout, True, fp64, seed=61, configure=cfg)
