@@ -185,10 +185,17 @@ template <typename Real, bool NT = false> struct LineIO {
   Real hw;  // interior half-spectrum weight (1: none)
 
   __device__ __forceinline__ C load(int32_t pos) const {
+    if (kind == kC2C) return load_kind<kC2C>(pos);
+    if (kind == kR2C) return load_kind<kR2C>(pos);
+    return load_kind<kC2R>(pos);
+  }
+  // the same for a kind known at compile time (a loop over positions then
+  // has no per-element branch on the kind)
+  template <int KIND> __device__ __forceinline__ C load_kind(int32_t pos) const {
     C v;
-    if (kind == kC2C) {
+    if constexpr (KIND == kC2C) {
       v = gload_c<NT, Real>(reinterpret_cast<const C*>(ib) + pos * is);
-    } else if (kind == kR2C) {
+    } else if constexpr (KIND == kR2C) {
       v = C{gload<NT>(reinterpret_cast<const Real*>(ib) + pos * is), Real(0)};
     } else {
       // the address is selected, not the load: loads under a branch each
@@ -413,8 +420,34 @@ __device__ __forceinline__ double2 twiddle_load(const double2* p) {
   return double2{w.x, w.y};
 }
 
+// The twiddle-table entries radix stage S of lane t multiplies by.  When a
+// lane needs at most four (a few registers), they are loaded ahead: the
+// stage loop issues them before the LDS exchange that precedes the stage, so
+// their latency passes under the exchange's barriers instead of stalling
+// every wave of the block right after them.
+template <typename Real, int N, int RM, int S> struct StageTwiddles {
+  using Cfg = FastCfg<Real, N, RM>;
+  static constexpr int r = Cfg::P.rad[S];
+  static constexpr int L = span_before(N, S);
+  static constexpr int NB = Cfg::R / r;
+  static constexpr int ts = N / (L * r);
+  static constexpr bool kAhead = L > 1 && NB <= 4;
+  double2 w[kAhead ? NB : 1];
+  __device__ __forceinline__ void load(int t, const double2* tw) {
+    if constexpr (kAhead) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) w[q] = twiddle_load(tw + ((t * NB + q) % L) * ts);
+    }
+  }
+  __device__ __forceinline__ double2 get(int t, const double2* tw, int q) const {
+    if constexpr (kAhead) return w[q];
+    else return twiddle_load(tw + ((t * NB + q) % L) * ts);
+  }
+};
+
 template <typename Real, int N, int RM, int S>
-__device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const double2* tw) {
+__device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const double2* tw,
+                                              const StageTwiddles<Real, N, RM, S>& pre) {
   using Cfg = FastCfg<Real, N, RM>;
   using C = cplx_t<Real>;
   constexpr int r = Cfg::P.rad[S];
@@ -424,11 +457,8 @@ __device__ __forceinline__ void stage_compute(cplx_t<Real>* v, int t, const doub
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     if constexpr (L > 1) {
-      const int beta = t * NB + q;
-      const int k = beta % L;
-      constexpr int ts = N / (L * r);
       double2 pw[8];
-      twiddle_powers<r>(twiddle_load(tw + k * ts), pw);
+      twiddle_powers<r>(pre.get(t, tw, q), pw);
 #pragma unroll
       for (int j = 1; j < r; ++j) v[q * r + j] = cmul(v[q * r + j], to_prec<C>(pw[j]));
     }
@@ -540,12 +570,20 @@ __device__ __forceinline__ void final_regroup(cplx_t<Real>* v, int t, void* line
 }
 
 template <typename Real, int N, int RM, bool SPLIT, int S> struct StageLoop {
-  __device__ static __forceinline__ void run(cplx_t<Real>* v, int t, const double2* tw, void* line) {
-    stage_compute<Real, N, RM, S>(v, t, tw);
+  __device__ static __forceinline__ void run(cplx_t<Real>* v, int t, const double2* tw, void* line,
+                                             const StageTwiddles<Real, N, RM, S>& pre) {
+    stage_compute<Real, N, RM, S>(v, t, tw, pre);
     if constexpr (S + 1 < FastCfg<Real, N, RM>::NST) {
+      StageTwiddles<Real, N, RM, S + 1> next;
+      next.load(t, tw);
       stage_exchange<Real, N, RM, SPLIT, S>(v, t, line);
-      StageLoop<Real, N, RM, SPLIT, S + 1>::run(v, t, tw, line);
+      StageLoop<Real, N, RM, SPLIT, S + 1>::run(v, t, tw, line, next);
     }
+  }
+  __device__ static __forceinline__ void run(cplx_t<Real>* v, int t, const double2* tw, void* line) {
+    StageTwiddles<Real, N, RM, S> pre;
+    pre.load(t, tw);
+    run(v, t, tw, line, pre);
   }
 };
 
@@ -558,7 +596,8 @@ template <typename Real, int N, int RM, bool SPLIT, int S> struct StageLoop {
 // real and imaginary halves (2 lines x N + N/16 + 1 floats per lane pair).
 // ---------------------------------------------------------------------------
 template <typename Real, int N, int RM, int S>
-__device__ __forceinline__ void stage_compute2(cplx_t<Real>* va, cplx_t<Real>* vb, int t, const double2* tw) {
+__device__ __forceinline__ void stage_compute2(cplx_t<Real>* va, cplx_t<Real>* vb, int t, const double2* tw,
+                                               const StageTwiddles<Real, N, RM, S>& pre) {
   using Cfg = FastCfg<Real, N, RM>;
   using C = cplx_t<Real>;
   constexpr int r = Cfg::P.rad[S];
@@ -568,11 +607,8 @@ __device__ __forceinline__ void stage_compute2(cplx_t<Real>* va, cplx_t<Real>* v
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     if constexpr (L > 1) {
-      const int beta = t * NB + q;
-      const int k = beta % L;
-      constexpr int ts = N / (L * r);
       double2 pw[8];
-      twiddle_powers<r>(twiddle_load(tw + k * ts), pw);
+      twiddle_powers<r>(pre.get(t, tw, q), pw);
 #pragma unroll
       for (int j = 1; j < r; ++j) {
         const C w = to_prec<C>(pw[j]);
@@ -624,9 +660,17 @@ __device__ __forceinline__ void lds_permute2(cplx_t<Real>* va, cplx_t<Real>* vb,
 template <typename Real, int N, int RM, int S> struct StageLoop2 {
   __device__ static __forceinline__ void run(cplx_t<Real>* va, cplx_t<Real>* vb, int t, const double2* tw, Real* la,
                                              Real* lb) {
+    StageTwiddles<Real, N, RM, S> pre;
+    pre.load(t, tw);
+    run(va, vb, t, tw, la, lb, pre);
+  }
+  __device__ static __forceinline__ void run(cplx_t<Real>* va, cplx_t<Real>* vb, int t, const double2* tw, Real* la,
+                                             Real* lb, const StageTwiddles<Real, N, RM, S>& pre) {
     using Cfg = FastCfg<Real, N, RM>;
-    stage_compute2<Real, N, RM, S>(va, vb, t, tw);
+    stage_compute2<Real, N, RM, S>(va, vb, t, tw, pre);
     if constexpr (S + 1 < Cfg::NST) {
+      StageTwiddles<Real, N, RM, S + 1> next;
+      next.load(t, tw);
       constexpr int r = Cfg::P.rad[S];
       constexpr int L = span_before(N, S);
       constexpr int R = Cfg::R;
@@ -640,7 +684,7 @@ template <typename Real, int N, int RM, int S> struct StageLoop2 {
             return (beta / L) * L * r + beta % L + j * L;
           },
           [t](int q, int j) { return t * NB2 + q + j * (N / r2); });
-      StageLoop2<Real, N, RM, S + 1>::run(va, vb, t, tw, la, lb);
+      StageLoop2<Real, N, RM, S + 1>::run(va, vb, t, tw, la, lb, next);
     }
   }
 };
@@ -870,12 +914,19 @@ __device__ __forceinline__ void fast_line_body(const LinePass& p, int32_t blk, i
       for (int j = 0; j < R; ++j) v[j] = C{Real(0), Real(0)};
     }
   } else if (valid) {
-    // R2C / C2R lines (one branch around the loads, as for C2C)
+    // R2C / C2R lines (one branch around the loads, as for C2C, and one per kind)
+    if (p.kind == kR2C) {
 #pragma unroll
-    for (int q = 0; q < NB0; ++q) {
-      const int beta = t * NB0 + q;
+      for (int q = 0; q < NB0; ++q) {
 #pragma unroll
-      for (int j = 0; j < r0; ++j) v[q * r0 + j] = io.load(beta + j * (N / r0));
+        for (int j = 0; j < r0; ++j) v[q * r0 + j] = io.template load_kind<kR2C>(t * NB0 + q + j * (N / r0));
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NB0; ++q) {
+#pragma unroll
+        for (int j = 0; j < r0; ++j) v[q * r0 + j] = io.template load_kind<kC2R>(t * NB0 + q + j * (N / r0));
+      }
     }
   } else {
 #pragma unroll
