@@ -432,15 +432,23 @@ template <typename Real, int N, int RM, int S> struct StageTwiddles {
   static constexpr int NB = Cfg::R / r;
   static constexpr int ts = N / (L * r);
   static constexpr bool kAhead = L > 1 && NB <= 4;
+  // more entries per lane (the last radix-2 stage of 16 values per thread):
+  // when NB divides L they are the consecutive entries k0 .. k0 + NB - 1,
+  // k0 = t*NB mod L, and entry k0 + q is entry k0 times entry q, the same for
+  // every lane (a scalar load): one entry per lane is loaded ahead
+  static constexpr bool kBase = L > 1 && NB > 4 && L % NB == 0;
   double2 w[kAhead ? NB : 1];
   __device__ __forceinline__ void load(int t, const double2* tw) {
     if constexpr (kAhead) {
 #pragma unroll
       for (int q = 0; q < NB; ++q) w[q] = twiddle_load(tw + ((t * NB + q) % L) * ts);
+    } else if constexpr (kBase) {
+      w[0] = twiddle_load(tw + ((t * NB) % L) * ts);
     }
   }
   __device__ __forceinline__ double2 get(int t, const double2* tw, int q) const {
     if constexpr (kAhead) return w[q];
+    else if constexpr (kBase) return q == 0 ? w[0] : cmul(w[0], twiddle_load(tw + q * ts));
     else return twiddle_load(tw + ((t * NB + q) % L) * ts);
   }
 };
