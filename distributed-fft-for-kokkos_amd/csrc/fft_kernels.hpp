@@ -64,6 +64,10 @@ struct LinePass {
   // a row's end has its second line in the padding (that element is then not
   // loaded)
   int32_t dual_split;
+  // 1 (one-line lanes-across-lines C2C passes from a padded scratch into odd
+  // unpadded output rows): lines enumerated over the real elements only, so a
+  // block writes whole, aligned output segments (see lines_linear_unpadded)
+  int32_t linear_lines;
   // FP64 R2C passes only (pair kernel): 1 = the line of field f (the batch
   // axis prod_axis, six fields) is the pointwise product u_I * u_J of two of
   // the three fields that the input descriptor holds along that axis, (I, J)

@@ -1427,6 +1427,30 @@ inline bool dual_lines_apply_impl(const LinePass& p, int B) { return dual_lines_
 // takes each line's addresses on their own (two-lines mode 2) and pairs the
 // stores where they are adjacent and 16-byte aligned.  Returns false or
 // rewrites q into that enumeration.
+// A lanes-across-lines C2C pass whose output rows (the fastest batch axis)
+// are unpadded and odd, read from a scratch with padded rows: its lines are
+// enumerated over the real elements only, so a block's consecutive lines are
+// consecutive output elements also across a row end (whole, aligned output
+// segments; each line takes its input address on its own).  False if the
+// input rows are not padded (nothing to change).
+inline bool lines_linear_unpadded(const LinePass& p, LinePass& q) {
+  if (p.kind != kC2C || !p.lines_fast || p.nbatch < 1 || p.line_begin != 0 || p.pad_real <= 0) return false;
+  const int j = p.nbatch - 1, fa = p.batch_axis[j];
+  for (const DDesc* d : {&p.in, &p.out}) {
+    if (d->layout_left) return false;
+    if (d->map[d->ndim - 1] != fa || d->ntopo[fa] != 1) return false;
+  }
+  if (p.out.pitch > 0 || p.out.ext[fa] % 2 == 0) return false;
+  if (p.pad_axis != fa || p.batch_ext[j] <= 0) return false;
+  q = p;
+  q.nlines = p.nlines / p.batch_ext[j] * p.pad_real;
+  q.batch_ext[j] = p.pad_real;
+  q.batch_div[j] = FastDiv(static_cast<uint32_t>(p.pad_real));
+  q.pad_real = 0;
+  q.pad_axis = 0;
+  return true;
+}
+
 inline bool dual_lines_linear(const LinePass& p, int B, LinePass& q) {
   if (!p.dual_lines || p.kind != kC2C || !p.lines_fast || p.nbatch < 1 || B < 2 || B % 2) return false;
   if (p.line_begin != 0) return false;
@@ -1510,6 +1534,10 @@ hipError_t launch_fast(LinePass p, hipStream_t s) {
     }
     LinePass q;
     if (dual_lines_linear(p, Bd, q)) return launch_dual<Real, N, RM>(q, Bd, s);
+  }
+  {
+    LinePass q;
+    if (p.linear_lines && lines_linear_unpadded(p, q)) p = q;
   }
   const bool split = split_available<Real, N>() && p.split_lds != 0;
   const size_t line_bytes = split ? line_lds_bytes<Real, N, RM, true>() : line_lds_bytes<Real, N, RM, false>();

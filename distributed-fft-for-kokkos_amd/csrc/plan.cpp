@@ -431,6 +431,7 @@ LinePass Plan::make_pass(const PassSpec& s, const DDesc& in, const DDesc& out, b
   p.swizzle = s.swz;
   p.split_lds = s.split;
   p.dual_lines = s.dual;
+  p.linear_lines = s.lin;
   p.tw = twiddles(s.n);
   p.lines_per_block = s.lines_per_block;
   p.values_per_thread = s.values_per_thread;
@@ -893,6 +894,7 @@ void Plan::set_option(const std::string& key, long value) {
   else if (field == "swz") passes[i].swz = static_cast<int>(value);
   else if (field == "split") passes[i].split = static_cast<int>(value);
   else if (field == "dual") passes[i].dual = static_cast<int>(value);
+  else if (field == "lin") passes[i].lin = static_cast<int>(value);
   else throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 
@@ -915,6 +917,7 @@ long Plan::get_option(const std::string& key) const {
   if (field == "swz") return passes[i].swz;
   if (field == "split") return passes[i].split;
   if (field == "dual") return passes[i].dual;
+  if (field == "lin") return passes[i].lin;
   throw Error(DFFT_ERROR_INVALID_ARGUMENT, "unknown option " + key);
 }
 

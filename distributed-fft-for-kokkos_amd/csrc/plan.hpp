@@ -31,6 +31,7 @@ struct PassSpec {
   int swz = 0;           // column-major walk of the block grid
   int split = 1;         // exchanges in real / imaginary halves (half the LDS; FP64, FP32 n >= 1024)
   int dual = 1;          // FP32 C2C: two adjacent lines per lane (16-byte accesses) where they apply
+  int lin = 1;           // one-line C2C passes into odd unpadded rows: lines enumerated over real elements
 };
 
 // device table exp(-2 pi i e / n), e in [0, n): evaluated in long double,
@@ -67,11 +68,12 @@ class Plan {
   int num_passes(int direction) const;
   // tuning knobs: "chunk" (planes per chunk of the first two passes, 0 = off),
   // "pad" (scratch rows padded to 128-byte multiples: -1 auto, 0 off, 1 on),
-  // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd|swz|split|dual", "bwd.<i>.…" (lines
+  // "fwd.<i>.lpb|vpt|lf|nt|pair|xcd|swz|split|dual|lin", "bwd.<i>.…" (lines
   // per block, values per thread, lanes-across-lines, nontemporal accesses,
   // two real lines per complex FFT in an R2C / C2R pass, XCD-contiguous
   // blocks, column-major block walk, LDS exchanges in real/imaginary halves,
-  // two adjacent FP32 lines per lane) of pass i
+  // two adjacent FP32 lines per lane, one-line passes into odd unpadded rows
+  // enumerated over the real elements) of pass i
   void set_option(const std::string& key, long value);
   long get_option(const std::string& key) const;
   // measure candidate launch shapes and chunk sizes on the caller's arrays

@@ -547,7 +547,7 @@ class Plan:
         out = {"fwd.chunk": self.get_option("fwd.chunk"), "bwd.chunk": self.get_option("bwd.chunk")}
         for d, dirn in (("fwd", Direction.forward), ("bwd", Direction.backward)):
             for i in range(max(0, self.num_passes(dirn))):
-                for f in ("lpb", "vpt", "lf", "nt", "pair", "xcd", "swz", "split", "dual"):
+                for f in ("lpb", "vpt", "lf", "nt", "pair", "xcd", "swz", "split", "dual", "lin"):
                     out[f"{d}.{i}.{f}"] = self.get_option(f"{d}.{i}.{f}")
         return out
 

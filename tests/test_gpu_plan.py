@@ -453,3 +453,26 @@ def test_unpaired_real_lines(dev, fp64, lf):
                                  ([16, 1024, 3], [0, 2, 1], [1, 2, 1], [2, 1, 1]),
                                  ([12, 16, 32], [0, 1, 2], [2, 2, 1], [1, 2, 2])):
         _check(dev, gin, axes, tin, tout, True, fp64, seed=61, configure=cfg)
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+@pytest.mark.parametrize("lin", [0, 1])
+def test_linear_line_enumeration(dev, fp64, lin):
+    """One-line lanes-across-lines C2C passes from a padded scratch into odd
+    unpadded output rows (the forward x pass of an R2C slab plan, rows of
+    n/2 + 1 elements), with the lines enumerated over the real elements
+    (lin 1: blocks cross row ends) or over the padded rows (lin 0), every
+    lines-per-block size, against the oracle."""
+    for lpb in (2, 8, 16):
+        def cfg(plan, ins, outs, lpb=lpb):
+            plan.set_option("pad", 1)
+            for d in ("fwd", "bwd"):
+                for i in range(plan.num_passes(0 if d == "fwd" else 1)):
+                    plan.set_option(f"{d}.{i}.lin", lin)
+                    plan.set_option(f"{d}.{i}.lf", 1)
+                    plan.set_option(f"{d}.{i}.dual", 0)
+                    plan.set_option(f"{d}.{i}.lpb", lpb)
+            assert plan.get_option("fwd.2.lin") == lin
+        _check(dev, [64, 12, 64], [0, 1, 2], [2, 1, 1], [1, 2, 1], True, fp64, seed=71, configure=cfg)
+        _check(dev, [32, 10, 30], [0, 1, 2], [2, 1, 1], [1, 2, 1], True, fp64, seed=72, configure=cfg)
+        _check(dev, [16, 8, 64], [0, 1, 2], [2, 2, 1], [1, 2, 2], True, fp64, seed=73, configure=cfg)
