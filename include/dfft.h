@@ -158,7 +158,11 @@ int dfft_execute_timed(dfft_plan* plan, void* const* in, void* const* out, int d
  * lines of 1024 points or more), and "fwd.<i>.dual" = two adjacent FP32
  * lines per lane with 16-byte accesses (default on; C2C lanes-across-lines
  * passes whose consecutive lines are adjacent row elements; elsewhere
- * ignored) of pass i. */
+ * ignored), "fwd.<i>.lin" = lines of a one-line C2C lanes-across-lines pass
+ * from the padded plan scratch into odd unpadded rows enumerated over the
+ * real elements, so blocks write whole aligned segments (default on;
+ * elsewhere ignored) of pass i.  A shape that does not fit a workgroup makes
+ * execute return DFFT_ERROR_HIP without launching. */
 int dfft_plan_set_option(dfft_plan* plan, const char* key, long value);
 int dfft_plan_get_option(const dfft_plan* plan, const char* key, long* value);
 /* Autotuner: times every launch shape of every pass and the chunk sizes on the
