@@ -52,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 N = 512
 P = 4
 # measured with rocprofv3 --pmc on this workload (profiles/fft_line_fast_512cubed_c2c_slab4.md)
-TRAFFIC_BYTES_PER_LAUNCH = 4.2958e9
+TRAFFIC_BYTES_PER_LAUNCH = 4.2956e9
 TRAFFIC_SOURCE = "profiles/fft_line_fast_512cubed_c2c_slab4.md: FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per launch"
 
 
