@@ -1460,15 +1460,10 @@ inline bool dual_lines_linear(const LinePass& p, int B, LinePass& q) {
     if (d->map[d->ndim - 1] != fa || d->ntopo[fa] != 1) return false;
   }
   if (p.out.pitch > 0 || p.out.ext[fa] % 2 == 0) return false;
+  // a padded input: the same enumeration over the real elements as the
+  // one-line kernel's; an unpadded one keeps its own
   q = p;
-  if (p.pad_real > 0) {
-    if (p.pad_axis != fa || p.batch_ext[j] <= 0) return false;
-    q.nlines = p.nlines / p.batch_ext[j] * p.pad_real;
-    q.batch_ext[j] = p.pad_real;
-    q.batch_div[j] = FastDiv(static_cast<uint32_t>(p.pad_real));
-    q.pad_real = 0;
-    q.pad_axis = 0;
-  }
+  if (p.pad_real > 0 && !lines_linear_unpadded(p, q)) return false;
   q.dual_split = 2;
   return true;
 }
